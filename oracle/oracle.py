@@ -1,0 +1,203 @@
+"""ctypes front end of the oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module, as the checker or the timed CPU baseline.  The product package
+gcdt_amd never imports it.
+
+Functions:
+  cdt(pts_grid, cons)            serial exact CDT (cdt_oracle.c)
+  validate(pts_grid, tris, cons) exact validator -> dict of counters
+  canonical(tris)                canonical triangle set (rotate smallest id
+                                 first, keep CCW, sort rows)
+  brute_force_dt(pts_grid)       O(n^4) empty-circumcircle enumeration on
+                                 tiny inputs (Python big ints), pins cdt()
+  orient2d / incircle            scalar predicates for golden-vector checks
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+GRID_R = (1 << 30) - 1
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        i64 = ctypes.c_longlong
+        L.oracle_orient2d.argtypes = [i64] * 6
+        L.oracle_orient2d.restype = ctypes.c_int
+        L.oracle_incircle.argtypes = [i64] * 8
+        L.oracle_incircle.restype = ctypes.c_int
+        vp = ctypes.c_void_p
+        L.oracle_cdt.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp]
+        L.oracle_cdt.restype = ctypes.c_int
+        L.oracle_validate.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int, vp]
+        L.oracle_validate.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def orient2d(a, b, c) -> int:
+    return lib().oracle_orient2d(int(a[0]), int(a[1]), int(b[0]), int(b[1]), int(c[0]), int(c[1]))
+
+
+def incircle(a, b, c, p) -> int:
+    return lib().oracle_incircle(int(a[0]), int(a[1]), int(b[0]), int(b[1]), int(c[0]), int(c[1]),
+                                 int(p[0]), int(p[1]))
+
+
+ERRORS = {0: "ok", 1: "alloc", 2: "duplicate point", 3: "point outside the open square",
+          4: "bad constraint ids", 5: "crossing constraints", 6: "internal"}
+
+
+def cdt(pts_grid, cons=None):
+    """Serial exact CDT.  pts_grid: (n,2) int32 on the 2^30 grid strictly inside
+    the square; cons: (c,2) vertex ids.  Returns (tris (T,3) int32 with corner ids
+    n..n+3, stats dict)."""
+    p = np.ascontiguousarray(pts_grid, dtype=np.int32).reshape(-1, 2)
+    n = len(p)
+    c = np.ascontiguousarray(cons if cons is not None else np.zeros((0, 2)), dtype=np.int32).reshape(-1, 2)
+    out = np.empty((2 * n + 2, 3), dtype=np.int32)
+    nt = ctypes.c_int(0)
+    st = np.zeros(4, dtype=np.int64)
+    rc = lib().oracle_cdt(p.ctypes.data, n, c.ctypes.data, len(c), out.ctypes.data, ctypes.byref(nt), st.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"oracle_cdt failed: {ERRORS.get(rc, rc)}")
+    return out[: nt.value].copy(), {"flips": int(st[0]), "splits": int(st[1]), "T": int(st[2]),
+                                    "cavity_flips": int(st[3])}
+
+
+VALIDATE_KEYS = ["not_ccw", "bad_ids", "area_mismatch", "edge_mismatch", "bad_boundary", "unused_vertices",
+                 "missing_constraints", "not_delaunay", "cocircular", "bad_count", "constraint_pieces",
+                 "edges_checked"]
+
+
+def validate(pts_grid, tris, cons=None):
+    """Exact validator; returns a dict (see cdt_oracle.c: oracle_validate).
+    d['valid'] is True iff the triangulation is a CDT of the input."""
+    p = np.ascontiguousarray(pts_grid, dtype=np.int32).reshape(-1, 2)
+    t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+    c = np.ascontiguousarray(cons if cons is not None else np.zeros((0, 2)), dtype=np.int32).reshape(-1, 2)
+    out = np.zeros(12, dtype=np.int64)
+    rc = lib().oracle_validate(p.ctypes.data, len(p), t.ctypes.data, len(t), c.ctypes.data, len(c), out.ctypes.data)
+    d = {k: int(v) for k, v in zip(VALIDATE_KEYS, out)}
+    d["valid"] = rc == 0
+    d["unique"] = rc == 0 and d["cocircular"] == 0
+    return d
+
+
+def canonical(tris) -> np.ndarray:
+    """Rotate each CCW triangle so its smallest id comes first, then sort rows."""
+    t = np.asarray(tris, dtype=np.int64).reshape(-1, 3)
+    r = np.argmin(t, axis=1)
+    idx = (r[:, None] + np.arange(3)[None, :]) % 3
+    rt = np.take_along_axis(t, idx, axis=1)
+    order = np.lexsort((rt[:, 2], rt[:, 1], rt[:, 0]))
+    return rt[order].astype(np.int32)
+
+
+def with_corners(pts_grid) -> np.ndarray:
+    p = np.asarray(pts_grid, dtype=np.int64).reshape(-1, 2)
+    c = np.array([[0, 0], [GRID_R, 0], [GRID_R, GRID_R], [0, GRID_R]], dtype=np.int64)
+    return np.concatenate([p, c])
+
+
+def _orient_py(a, b, c):
+    v = (a[0] - c[0]) * (b[1] - c[1]) - (a[1] - c[1]) * (b[0] - c[0])
+    return (v > 0) - (v < 0)
+
+
+def _incircle_py(a, b, c, d):
+    """sign of det[[ax-dx, ay-dy, |a-d|^2], ...] with Python big ints."""
+    m = []
+    for q in (a, b, c):
+        x, y = q[0] - d[0], q[1] - d[1]
+        m.append((x, y, x * x + y * y))
+    det = (m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1])
+           - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0])
+           + m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]))
+    return (det > 0) - (det < 0)
+
+
+def brute_force_dt(pts_grid) -> np.ndarray:
+    """All CCW triangles of P + corners whose open circumdisk is empty.
+    O(n^4); n <= ~40.  In general position this is the Delaunay triangulation."""
+    P = [tuple(int(v) for v in q) for q in with_corners(pts_grid)]
+    n = len(P)
+    out = []
+    for a, b, c in itertools.combinations(range(n), 3):
+        o = _orient_py(P[a], P[b], P[c])
+        if o == 0:
+            continue
+        tri = (a, b, c) if o > 0 else (a, c, b)
+        if all(_incircle_py(P[tri[0]], P[tri[1]], P[tri[2]], P[d]) <= 0 for d in range(n) if d not in tri):
+            out.append(tri)
+    return canonical(np.array(out, dtype=np.int64).reshape(-1, 3))
+
+
+def _segments_cross(p, q, r, s):
+    """proper crossing of open segments pq and rs (exact)."""
+    o1, o2 = _orient_py(p, q, r), _orient_py(p, q, s)
+    o3, o4 = _orient_py(r, s, p), _orient_py(r, s, q)
+    return o1 * o2 < 0 and o3 * o4 < 0
+
+
+def brute_force_cdt(pts_grid, cons) -> np.ndarray:
+    """Constrained Delaunay triangles by definition on tiny inputs: a CCW
+    triangle is kept iff no constraint properly crosses its edges, no vertex lies
+    inside it, and no vertex strictly inside its circumcircle is visible from
+    its centroid (the sight line crosses no constraint).  Constraints must not
+    pass through other vertices."""
+    P = [tuple(int(v) for v in q) for q in with_corners(pts_grid)]
+    # scale by 3 so the centroid is integral
+    P3 = [(3 * x, 3 * y) for x, y in P]
+    C = [(int(a), int(b)) for a, b in np.asarray(cons).reshape(-1, 2)]
+    n = len(P)
+    out = []
+    for a, b, c in itertools.combinations(range(n), 3):
+        o = _orient_py(P[a], P[b], P[c])
+        if o == 0:
+            continue
+        tri = (a, b, c) if o > 0 else (a, c, b)
+        edges = [(tri[0], tri[1]), (tri[1], tri[2]), (tri[2], tri[0])]
+        bad = False
+        for (u, v) in edges:
+            for (s, t) in C:
+                if _segments_cross(P[u], P[v], P[s], P[t]):
+                    bad = True
+                    break
+            if bad:
+                break
+        if bad:
+            continue
+        g = (P[tri[0]][0] + P[tri[1]][0] + P[tri[2]][0], P[tri[0]][1] + P[tri[1]][1] + P[tri[2]][1])
+        for d in range(n):
+            if d in tri:
+                continue
+            if all(_orient_py(P[e[0]], P[e[1]], P[d]) >= 0 for e in edges):
+                bad = True  # a vertex inside or on the triangle
+                break
+            if _incircle_py(P[tri[0]], P[tri[1]], P[tri[2]], P[d]) > 0:
+                visible = not any(_segments_cross(P3[d], g, P3[s], P3[t]) for (s, t) in C)
+                if visible:
+                    bad = True
+                    break
+        if not bad:
+            out.append(tri)
+    return canonical(np.array(out, dtype=np.int64).reshape(-1, 3))
