@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: full constrained Delaunay triangulation on one MI355X.
+
+Workload (one step): the 10M-point plain CDT -- 10,000,000 int32 draws in
+[0,2^20)^2, 80% uniform and 20% Gaussian clusters (sigma 2^12, 64 centres),
+rounded and deduplicated, plus 1,000 disjoint circles, each a 100-gon with
+vertices snapped to the grid (100,000 constraint segments).  Inputs are
+generated once (seeded SplitMix64), mapped exactly onto the 2^30 grid and
+uploaded; each timed step is one gcdt_solve from device-resident inputs to a
+device-resident triangle list.
+
+Prints one JSON line: value = Mpts/s (points / solve wall time), roofline of
+the dominant kernel (live hipEvent timing inside this process), the exact
+validity of the last solve, and the CPU baseline (the serial exact-predicate
+oracle on the same workload, one host core).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# Algorithmic bytes per processed list item of the instrumented kernels.
+#   flip_test : tri 12 + adj 12 + cflag 4 + own 3 points 24
+#               + 3 x (neighbour tri 12 + opposite point 8) + cand 4 + 2 claims 16
+#   ins_reloc : loc 4 + sons 4 + list write 4 + 2 children tri 24 + 3 points 24 + loc write 4
+#   ins_claim : loc 4 + point id 4 + 1.x claim atomics 8
+#   flip_apply: cand 4 + tri 12 + adj 12 + 2 claims 16 (winners add 2 x 28 write)
+#   locate_bitmap: point 8 + isv 4 + 2 offsets 8 + ~1.5 x (tri id 4 + tri 12 + 3 points 24) + loc 4 + list 4
+BYTES_PER_ITEM = {
+    "flip_test": 132,
+    "ins_reloc": 64,
+    "ins_claim": 16,
+    "flip_apply": 44,
+    "locate_bitmap": 88,
+    "ins_split": 16,
+}
+
+
+def _prof(lib, name):
+    import ctypes
+    ms = ctypes.c_double(0)
+    launches = ctypes.c_int64(0)
+    items = ctypes.c_int64(0)
+    if lib.gcdt_prof_read(name.encode(), ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(items)) != 0:
+        return None
+    return ms.value, launches.value, items.value
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--points", type=int, default=10_000_000)
+    ap.add_argument("--circles", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-validate", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every GPU box here has one MI355X, and this CDT path has no "
+              "multi-GPU code (the sharding design is documented, not built). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from gcdt_amd import DeviceBuffer, gen, lib, solve_device
+    from gcdt_amd.lib import check
+
+    L = lib()
+    t0 = time.time()
+    pts, cons = gen.clustered_cdt(n=args.points, circles=args.circles)
+    g = gen.to_grid(pts)
+    n, c = len(g), len(cons)
+    print(f"# workload: {n} points, {c} segments (generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+
+    dp = DeviceBuffer.from_array(g)
+    dc = DeviceBuffer.from_array(cons)
+    dt = DeviceBuffer(12 * (2 * n + 2))
+    da = DeviceBuffer(12 * (2 * n + 2))
+    for i in range(args.warmup):
+        T, st = solve_device(dp, n, dc, c, dt, da)
+        print(f"# warmup {i}: {st['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+    check(L.gcdt_synchronize(), "sync")
+    check(L.gcdt_prof_reset(), "prof")
+    check(L.gcdt_prof_enable(1), "prof")
+    steps = []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        T, st = solve_device(dp, n, dc, c, dt, da)
+        steps.append(st)
+        print(f"# step {i}: {st['ms_total']:.1f} ms (insert {st['ms_insert']:.1f}, flip {st['ms_flip']:.1f}, "
+              f"cons {st['ms_constraints']:.1f}, sample {st['ms_sample']:.1f}, bitmap {st['ms_bitmap']:.1f}; "
+              f"rounds ins {st['insert_rounds']} flip {st['flip_rounds']} cons {st['constraint_rounds']})",
+              file=sys.stderr, flush=True)
+    check(L.gcdt_synchronize(), "sync")
+    t_end = time.perf_counter()
+    check(L.gcdt_prof_enable(0), "prof")
+    ms_per_step = (t_end - t_start) * 1000.0 / args.steps
+    mpts = n / (ms_per_step / 1000.0) / 1e6
+
+    # dominant instrumented kernel and its roofline
+    kern = {}
+    for name in BYTES_PER_ITEM:
+        r = _prof(L, name)
+        if r:
+            kern[name] = r
+    dom = max(kern, key=lambda k: kern[k][0]) if kern else None
+    roof = None
+    if dom:
+        ms, launches, items = kern[dom]
+        avg_ms = ms / max(launches, 1)
+        bytes_per_launch = BYTES_PER_ITEM[dom] * items / max(launches, 1)
+        achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
+        roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "avg_launch_us": round(avg_ms * 1000.0, 3), "launches_per_step": launches // args.steps,
+                "kernel_ms_per_step": round(ms / args.steps, 3),
+                "bytes_per_item": BYTES_PER_ITEM[dom], "items_per_launch": items // max(launches, 1)}
+    kernels = {k: {"ms_per_step": round(v[0] / args.steps, 3), "launches_per_step": v[1] // args.steps,
+                   "items_per_step": v[2] // args.steps} for k, v in kern.items()}
+    # whole-solve algorithmic bytes: points in, segments in, triangles out
+    b_alg = 8 * n + 8 * c + 12 * T
+    solve_roof = {"bytes": b_alg, "achieved_GBs": round(b_alg / (ms_per_step / 1000.0) / 1e9, 3),
+                  "frac": round(b_alg / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 6)}
+
+    valid = None
+    if not args.no_validate:
+        from oracle import oracle as O
+        tris = dt.to_array(np.int32, (T, 3))
+        v = O.validate(g, tris, cons)
+        valid = {"valid": v["valid"], "not_delaunay": v["not_delaunay"],
+                 "missing_constraints": v["missing_constraints"], "cocircular_edges": v["cocircular"]}
+        print(f"# validation: {v}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        tc = time.perf_counter()
+        O.cdt(g, cons)
+        dtc = time.perf_counter() - tc
+        cpu = {"value": round(n / dtc / 1e6, 4), "unit": "Mpts/s", "cores": 1, "kind": "port",
+               "sample": f"the full workload ({n} points, {c} segments), serial exact-predicate CDT, "
+                         f"one host core, {dtc:.2f} s"}
+
+    out = {
+        "metric": "Mpts/s CDT at 10M pts; batched CDTs/s vs per-instance launch; p99 latency",
+        "value": round(mpts, 4),
+        "unit": "Mpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64/int128 (exact integer predicates)",
+        "data": "synthetic (seeded SplitMix64 generator)",
+        "config": {"workload": "plain CDT, large: 10M int32 points (80% uniform, 20% Gaussian clusters "
+                               "sigma=2^12, 64 centres) + 1000 disjoint 100-gon circles (100k segments)",
+                   "points": n, "segments": c, "triangles": T, "parallelism": "single GPU"},
+        "roofline": roof,
+        "solve_roofline": solve_roof,
+        "kernels": kernels,
+        "validity": valid,
+        "last_step": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in steps[-1].items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    for b in (dp, dc, dt, da):
+        b.free()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,10 @@
+"""gcdt_amd: MI355X (gfx950) constrained Delaunay triangulation.
+
+The compute path is libgcdt.so (HIP kernels behind the C ABI of
+include/gcdt.h).  This package only loads it and moves host arrays in and out;
+if the library is missing every call raises -- there is no CPU fallback.
+"""
+from .lib import GcdtError, lib, load_error  # noqa: F401
+from .api import DeviceBuffer, solve, solve_device  # noqa: F401
+
+__all__ = ["GcdtError", "lib", "load_error", "DeviceBuffer", "solve", "solve_device"]

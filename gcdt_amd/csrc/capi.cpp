@@ -1,0 +1,629 @@
+// C ABI of the library: runtime helpers, the reference's entry points and the
+// one-call solver.  Every function catches exceptions at the boundary and
+// returns a GCDT_* status.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "mesh.hpp"
+
+using namespace gcdt;
+
+static thread_local std::string g_last_error;
+
+static int fail(int status, const std::string& msg)
+{
+    g_last_error = msg;
+    return status;
+}
+
+#define API_BEGIN try {
+#define API_END                                             \
+    }                                                       \
+    catch (const Error& e) { return fail(e.status, e.what()); } \
+    catch (const std::exception& e) { return fail(GCDT_ERR_INTERNAL, e.what()); }
+
+// hipEvent-timed region on a stream
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t st;
+    explicit Timer(hipStream_t s) : st(s)
+    {
+        GCDT_HIP(hipEventCreate(&a));
+        GCDT_HIP(hipEventCreate(&b));
+        GCDT_HIP(hipEventRecord(a, st));
+    }
+    double ms()
+    {
+        GCDT_HIP(hipEventRecord(b, st));
+        GCDT_HIP(hipEventSynchronize(b));
+        float f = 0;
+        GCDT_HIP(hipEventElapsedTime(&f, a, b));
+        return f;
+    }
+    ~Timer()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+extern "C" {
+
+const char* gcdt_last_error(void) { return g_last_error.c_str(); }
+
+int gcdt_device_count(int* count)
+{
+    API_BEGIN
+    int n = 0;
+    GCDT_HIP(hipGetDeviceCount(&n));
+    if (count) *count = n;
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_synchronize(void)
+{
+    API_BEGIN
+    GCDT_HIP(hipDeviceSynchronize());
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_alloc(void** ptr, size_t bytes)
+{
+    API_BEGIN
+    void* p = getMemCacheRef().getMemByAbtrtSize(bytes);
+    if (!p) return fail(GCDT_ERR_OOM, "pool allocation failed");
+    *ptr = p;
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_release(void* ptr, size_t bytes)
+{
+    API_BEGIN
+    getMemCacheRef().releaseAbtrtMem(ptr, bytes);
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_pool_init(void)
+{
+    API_BEGIN
+    getMemCacheRef().initAbtrtMem();
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_pool_stats(size_t* in_use, size_t* cached, size_t* peak)
+{
+    MemCache& m = getMemCacheRef();
+    if (in_use) *in_use = m.bytesInUse();
+    if (cached) *cached = m.bytesCached();
+    if (peak) *peak = m.peakBytes();
+    return GCDT_OK;
+}
+
+int gcdt_pool_set_budget(size_t bytes)
+{
+    getMemCacheRef().setBudget(bytes);
+    return GCDT_OK;
+}
+
+int gcdt_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    API_BEGIN
+    GCDT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    API_BEGIN
+    GCDT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_memcpy_d2d(void* dst, const void* src, size_t bytes)
+{
+    API_BEGIN
+    GCDT_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_memset(void* dst, int value, size_t bytes)
+{
+    API_BEGIN
+    GCDT_HIP(hipMemset(dst, value, bytes));
+    return GCDT_OK;
+    API_END
+}
+
+size_t gcdt_scratch_bytes(int64_t n) { return sizeof(int) * (size_t)scan_scratch_ints(n); }
+
+int gcdt_prof_enable(int on)
+{
+    prof_enable(on != 0);
+    return GCDT_OK;
+}
+
+int gcdt_prof_reset(void)
+{
+    prof_reset();
+    return GCDT_OK;
+}
+
+int gcdt_prof_read(const char* kernel, double* total_ms, int64_t* launches, int64_t* items)
+{
+    long long l = 0, it = 0;
+    double ms = 0;
+    if (!prof_read(kernel, &ms, &l, &it)) return fail(GCDT_ERR_INVALID, "kernel not profiled");
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = l;
+    if (items) *items = it;
+    return GCDT_OK;
+}
+
+int gcdt_prof_names(char* buf, int cap) { return prof_names(buf, cap); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ the solve
+
+namespace gcdt {
+
+struct SolveOut {
+    int T = 0;
+    gcdt_stats st{};
+};
+
+// Full pipeline on device arrays; pts holds N+4 entries (corners written here).
+static void solve_impl(Ctx& c, int2* pts, int N, const int2* cons, int C, int3* tris_out, int3* adj_out,
+                       SolveOut& out)
+{
+    gcdt_stats& st = out.st;
+    Timer total(c.st);
+    check_points(c, pts, N);
+    write_corners(c, pts, N);
+    // ---- 1. sample + sample mesh
+    Timer t1(c.st);
+    DBuf<int> sid(1 << 16);
+    int S = sample_points(c, pts, N, sid.p, nullptr);
+    DBuf<int> loc((size_t)N + 4), pend((size_t)(N > 1 ? N : 1)), pend2((size_t)(N > 1 ? N : 1));
+    long long capA = 4LL * S + 8;
+    DBuf<int3> tA(capA), aA(capA);
+    DBuf<int> sA(capA);
+    DBuf<unsigned long long> clA(capA);
+    init_square(c, nullptr, tA.p, aA.p, N);
+    fill_i32(c, sA.p, -1, capA);
+    fill_u64(c, clA.p, 0, capA);
+    locate_two_ids(c, pts, sid.p, S, loc.p, pend.p);
+    Ins ia;
+    ia.m = Mesh{pts, tA.p, aA.p, sA.p, nullptr};
+    ia.loc = loc.p;
+    ia.claim = clA.p;
+    ia.cap = (int)capA;
+    ia.err = c.dcnt.p + C_ERR;
+    int rs = 0;
+    int ntA = insert_rounds(c, ia, pend, pend2, S, 2, &rs);
+    long long capB = 4LL * N + 8;
+    DBuf<int3> tB(capB), aB(capB);
+    int Ts = filter_leaves(c, tA.p, aA.p, sA.p, ntA, tB.p, aB.p, capB);
+    tA.release();
+    aA.release();
+    sA.release();
+    clA.release();
+    int fr = 0;
+    st.flips += flip_rounds(c, Mesh{pts, tB.p, aB.p, nullptr, nullptr}, Ts, nullptr, 0, &fr);
+    st.ms_sample = t1.ms();
+    // ---- 2. bitmap over the sample Delaunay mesh
+    Timer t2(c.st);
+    int *bo = nullptr, *bt = nullptr, nbt = 0;
+    build_bitmap(c, pts, tB.p, Ts, &bo, &bt, &nbt);
+    st.ms_bitmap = t2.ms();
+    // ---- 3. locate + insert the rest
+    Timer t3(c.st);
+    DBuf<int> isv(N > 0 ? N : 1);
+    fill_i32(c, isv.p, 0, N);
+    int np = locate_bitmap(c, pts, N, tB.p, Ts, bo, bt, loc.p, isv.p, pend.p);
+    getMemCacheRef().releaseAbtrtMem(bo, sizeof(int) * ((1 << 20) + 1));
+    getMemCacheRef().releaseAbtrtMem(bt, sizeof(int) * (size_t)(nbt > 0 ? nbt : 1));
+    isv.release();
+    DBuf<int> sB(capB);
+    DBuf<unsigned long long> clB(capB);
+    fill_i32(c, sB.p, -1, capB);
+    fill_u64(c, clB.p, 0, capB);
+    Ins ib;
+    ib.m = Mesh{pts, tB.p, aB.p, sB.p, nullptr};
+    ib.loc = loc.p;
+    ib.claim = clB.p;
+    ib.cap = (int)capB;
+    ib.err = c.dcnt.p + C_ERR;
+    int ri = 0;
+    int ntB = insert_rounds(c, ib, pend, pend2, np, Ts, &ri);
+    st.insert_rounds = rs + ri;
+    clB.release();
+    loc.release();
+    pend.release();
+    pend2.release();
+    int T = filter_leaves(c, tB.p, aB.p, sB.p, ntB, tris_out, adj_out, 2LL * N + 2);
+    tB.release();
+    aB.release();
+    sB.release();
+    if (T != 2 * N + 2) throw Error(GCDT_ERR_INTERNAL, "leaf count " + std::to_string(T) + " != 2N+2");
+    st.ms_insert = t3.ms();
+    // ---- 4. Delaunay flips
+    Timer t4(c.st);
+    int f2 = 0;
+    st.flips += flip_rounds(c, Mesh{pts, tris_out, adj_out, nullptr, nullptr}, T, nullptr, 0, &f2);
+    st.flip_rounds = fr + f2;
+    st.ms_flip = t4.ms();
+    // ---- 5. constraints
+    Timer t5(c.st);
+    if (C > 0) {
+        DBuf<int> cflag(T);
+        fill_i32(c, cflag.p, 0, T);
+        Mesh m{pts, tris_out, adj_out, nullptr, cflag.p};
+        ConsResult cr = insert_constraints(c, m, T, N + 4, cons, C, nullptr, nullptr);
+        st.constraint_rounds = cr.rounds;
+        st.segments = cr.pieces;
+        st.splits = cr.pieces - C;
+        int f3 = 0;
+        st.flips += flip_rounds(c, m, T, nullptr, 0, &f3);
+        st.flip_rounds += f3;
+    }
+    st.ms_constraints = t5.ms();
+    st.ms_total = total.ms();
+    st.num_triangles = T;
+    out.T = T;
+}
+
+}  // namespace gcdt
+
+extern "C" int gcdt_solve(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcdt_int3* tris_out,
+                          gcdt_int3* adj_out, int* num_tris, gcdt_stats* stats, void* stream)
+{
+    API_BEGIN
+    if (N < 0 || C < 0 || (N > 0 && !points) || (C > 0 && !cons) || !tris_out)
+        return fail(GCDT_ERR_INVALID, "bad arguments");
+    if (N > (1 << 28)) return fail(GCDT_ERR_INVALID, "too many points");
+    hipStream_t st = (hipStream_t)stream;
+    Ctx c(st);
+    DBuf<int2> pts((size_t)N + 4);
+    if (N > 0) GCDT_HIP(hipMemcpyAsync(pts.p, points, sizeof(int2) * (size_t)N, hipMemcpyDeviceToDevice, st));
+    DBuf<int3> adj_tmp;
+    int3* adj = (int3*)adj_out;
+    if (!adj) {
+        adj_tmp.alloc(2LL * N + 2);
+        adj = adj_tmp.p;
+    }
+    SolveOut out;
+    solve_impl(c, pts.p, N, (const int2*)cons, C, (int3*)tris_out, adj, out);
+    if (num_tris) *num_tris = out.T;
+    if (stats) *stats = out.st;
+    return GCDT_OK;
+    API_END
+}
+
+// ------------------------------------------------------------------ reference entry points
+
+namespace {
+
+// Ctx on the legacy default stream with the caller's scratch attached.
+struct ApiCtx {
+    Ctx c;
+    ApiCtx(void* tmp, size_t bytes) : c(nullptr)
+    {
+        if (tmp) {
+            c.ext_scratch = (int*)tmp;
+            c.ext_scratch_bytes = bytes;
+        }
+    }
+};
+
+__global__ void k_convert_sample(int3* tris, const int* sid, int ns, int np, int T)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        int3 t = tris[i];
+        t.x = t.x >= ns ? t.x - ns + np : sid[t.x];
+        t.y = t.y >= ns ? t.y - ns + np : sid[t.y];
+        t.z = t.z >= ns ? t.z - ns + np : sid[t.z];
+        tris[i] = t;
+    }
+}
+
+__global__ void k_retire(const int* touched, const int* leaf_slot, int T, int* sons)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x)
+        if (touched[i]) sons[leaf_slot[i]] = 0;
+}
+
+__global__ void k_leaf_slots(const int* sons, const int* off, int T, int* leaf_slot)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x)
+        if (sons[i] == -1) leaf_slot[off[i]] = (int)i;
+}
+
+__global__ void k_flags_leaf(const int* sons, int T, int* f)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x)
+        f[i] = sons[i] == -1;
+}
+
+__global__ void k_gather_touched(const int3* tri, const int* touched, const int* off, int T, int3* out)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x)
+        if (touched[i]) out[off[i]] = tri[i];
+}
+
+// Insertion rounds from the two corner triangles over all points, with
+// internal adjacency when the caller gives none.
+static void raw_mesh_impl(Ctx& c, int2* pts, int3* tris, int3* adj_out, int* sons, int* numT, int n)
+{
+    long long cap = 4LL * n + 4;
+    DBuf<int3> adj_tmp;
+    int3* adj = adj_out;
+    if (!adj) {
+        adj_tmp.alloc(cap);
+        adj = adj_tmp.p;
+    }
+    check_points(c, pts, n);
+    init_square(c, pts, tris, adj, n);
+    fill_i32(c, sons, -1, cap);
+    DBuf<unsigned long long> claim(cap);
+    fill_u64(c, claim.p, 0, cap);
+    DBuf<int> loc((size_t)n + 4), pend(n > 0 ? n : 1), pend2(n > 0 ? n : 1);
+    locate_two(c, pts, n, loc.p, pend.p);
+    Ins s;
+    s.m = Mesh{pts, tris, adj, sons, nullptr};
+    s.loc = loc.p;
+    s.claim = claim.p;
+    s.cap = (int)cap;
+    s.err = c.dcnt.p + C_ERR;
+    int r = 0;
+    *numT = insert_rounds(c, s, pend, pend2, n, 2, &r);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gcdt_get_sample_points(void* tmp, size_t tmpBytes, const gcdt_int2* points, int numPoints,
+                           gcdt_int2** samplePoints, int** sample, int* num, double* ms)
+{
+    API_BEGIN
+    if (numPoints < 0 || (numPoints > 0 && !points) || !samplePoints || !sample || !num)
+        return fail(GCDT_ERR_INVALID, "bad arguments");
+    ApiCtx a(tmp, tmpBytes);
+    Timer t(a.c.st);
+    check_points(a.c, (const int2*)points, numPoints);
+    const size_t cap = (1 << 16) + 4;  // rawMesh.cu:715-717
+    DBuf<int> ids(cap);
+    DBuf<int2> sp(cap);
+    int S = sample_points(a.c, (const int2*)points, numPoints, ids.p, sp.p);
+    if (ms) *ms = t.ms();
+    *num = S;
+    *sample = ids.detach();
+    *samplePoints = (gcdt_int2*)sp.detach();
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_raw_mesh_before(void* tmp, size_t tmpBytes, gcdt_int2* points, gcdt_int3* tris, int* sons,
+                         int* numTriangles, int numPoints, double* ms)
+{
+    API_BEGIN
+    if (numPoints < 0 || !points || !tris || !sons || !numTriangles) return fail(GCDT_ERR_INVALID, "bad arguments");
+    ApiCtx a(tmp, tmpBytes);
+    Timer t(a.c.st);
+    raw_mesh_impl(a.c, (int2*)points, (int3*)tris, nullptr, sons, numTriangles, numPoints);
+    if (ms) *ms = t.ms();
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_raw_mesh(void* tmp, size_t tmpBytes, gcdt_int2* points, gcdt_int3* tris, gcdt_int3* adjTris, int* sons,
+                  int* numTriangles, int numPoints, double* ms)
+{
+    API_BEGIN
+    if (numPoints < 0 || !points || !tris || !adjTris || !sons || !numTriangles)
+        return fail(GCDT_ERR_INVALID, "bad arguments");
+    ApiCtx a(tmp, tmpBytes);
+    Timer t(a.c.st);
+    raw_mesh_impl(a.c, (int2*)points, (int3*)tris, (int3*)adjTris, sons, numTriangles, numPoints);
+    if (ms) *ms = t.ms();
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_raw_mesh_after(void* tmp, size_t tmpBytes, gcdt_int2* points, gcdt_int3* tris, gcdt_int3* adjTris,
+                        int* sons, const int* bitmapOffsets, const int* bitmapTriIds, int* numTriangles,
+                        int numPoints, double* ms)
+{
+    API_BEGIN
+    if (numPoints < 0 || !points || !tris || !adjTris || !sons || !bitmapOffsets || !bitmapTriIds || !numTriangles ||
+        *numTriangles < 2)
+        return fail(GCDT_ERR_INVALID, "bad arguments");
+    ApiCtx a(tmp, tmpBytes);
+    Ctx& c = a.c;
+    Timer t(c.st);
+    int n = numPoints;
+    long long cap = 4LL * n + 4;
+    int T0 = *numTriangles;
+    check_points(c, (const int2*)points, n);
+    write_corners(c, (int2*)points, n);
+    fill_i32(c, sons, -1, cap);
+    DBuf<int> loc((size_t)n + 4), pend(n > 0 ? n : 1), pend2(n > 0 ? n : 1), isv(n > 0 ? n : 1);
+    fill_i32(c, isv.p, 0, n);
+    int np = locate_bitmap(c, (const int2*)points, n, (const int3*)tris, T0, bitmapOffsets, bitmapTriIds, loc.p,
+                           isv.p, pend.p);
+    DBuf<unsigned long long> claim(cap);
+    fill_u64(c, claim.p, 0, cap);
+    Ins s;
+    s.m = Mesh{(const int2*)points, (int3*)tris, (int3*)adjTris, sons, nullptr};
+    s.loc = loc.p;
+    s.claim = claim.p;
+    s.cap = (int)cap;
+    s.err = c.dcnt.p + C_ERR;
+    int r = 0;
+    *numTriangles = insert_rounds(c, s, pend, pend2, np, T0, &r);
+    if (ms) *ms = t.ms();
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_get_bitmap(void* tmp, size_t tmpBytes, const gcdt_int2* points, const gcdt_int3* tris, int** bitmapOffsets,
+                    int** bitmapTris, int numTriangles, double* ms)
+{
+    API_BEGIN
+    if (numTriangles < 0 || !points || !tris || !bitmapOffsets || !bitmapTris) return fail(GCDT_ERR_INVALID, "bad arguments");
+    ApiCtx a(tmp, tmpBytes);
+    Timer t(a.c.st);
+    int nb = 0;
+    build_bitmap(a.c, (const int2*)points, (const int3*)tris, numTriangles, bitmapOffsets, bitmapTris, &nb);
+    if (ms) *ms = t.ms();
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_filter_triangles(void* tmp, size_t tmpBytes, const gcdt_int3* tris, gcdt_int3* newTris,
+                          const gcdt_int3* adjTris, gcdt_int3* newAdjTris, const int* sons, int* numTriangles,
+                          double* ms)
+{
+    API_BEGIN
+    if (!tris || !newTris || !sons || !numTriangles || *numTriangles < 0) return fail(GCDT_ERR_INVALID, "bad arguments");
+    ApiCtx a(tmp, tmpBytes);
+    Timer t(a.c.st);
+    *numTriangles = filter_leaves(a.c, (const int3*)tris, (adjTris && newAdjTris) ? (const int3*)adjTris : nullptr,
+                                  sons, *numTriangles, (int3*)newTris, newAdjTris ? (int3*)newAdjTris : nullptr,
+                                  (long long)*numTriangles);
+    if (ms) *ms = t.ms();
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_convert_sample(gcdt_int3* tris, const int* sampleId, int numSamples, int numPoints, int numTriangles)
+{
+    API_BEGIN
+    if (!tris || !sampleId || numTriangles < 0) return fail(GCDT_ERR_INVALID, "bad arguments");
+    if (numTriangles > 0) {
+        k_convert_sample<<<grid_for(numTriangles), kBlock, 0, nullptr>>>((int3*)tris, sampleId, numSamples, numPoints,
+                                                                         numTriangles);
+        GCDT_LAUNCH_CHECK();
+        GCDT_HIP(hipStreamSynchronize(nullptr));
+    }
+    return GCDT_OK;
+    API_END
+}
+
+static int flip_plain(void* tmp, size_t tmpBytes, const gcdt_int2* points, gcdt_int3* tris, gcdt_int3* adjTris,
+                      int numPoints, int numTriangles, double* ms)
+{
+    API_BEGIN
+    if (!points || !tris || numTriangles < 0 || numPoints < 3) return fail(GCDT_ERR_INVALID, "bad arguments");
+    ApiCtx a(tmp, tmpBytes);
+    Ctx& c = a.c;
+    Timer t(c.st);
+    DBuf<int3> adj_tmp;
+    int3* adj = (int3*)adjTris;
+    if (!adj) {
+        adj_tmp.alloc(numTriangles > 0 ? numTriangles : 1);
+        adj = adj_tmp.p;
+    }
+    build_adjacency(c, (const int3*)tris, numTriangles, numPoints, adj);
+    int r = 0;
+    flip_rounds(c, Mesh{(const int2*)points, (int3*)tris, adj, nullptr, nullptr}, numTriangles, nullptr, 0, &r);
+    if (ms) *ms = t.ms();
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_delaunay_sample(void* tmp, size_t tmpBytes, const gcdt_int2* points, gcdt_int3* tris, gcdt_int3* adjTris,
+                         int numPoints, int numTriangles, double* ms)
+{
+    return flip_plain(tmp, tmpBytes, points, tris, adjTris, numPoints, numTriangles, ms);
+}
+
+int gcdt_try_fast_delaunay(void* tmp, size_t tmpBytes, const gcdt_int2* points, gcdt_int3* tris, gcdt_int3* adjTris,
+                           int numPoints, int numTriangles, double* ms)
+{
+    return flip_plain(tmp, tmpBytes, points, tris, adjTris, numPoints, numTriangles, ms);
+}
+
+int gcdt_delaunay_new(void* tmp, size_t tmpBytes, gcdt_int2** points, gcdt_int3** tris, const gcdt_int2* cons,
+                      int numPoints, int numTriangles, int numCons, double* ms)
+{
+    API_BEGIN
+    if (!points || !*points || !tris || !*tris || numTriangles < 0 || numPoints < 3 || numCons < 0 ||
+        (numCons > 0 && !cons))
+        return fail(GCDT_ERR_INVALID, "bad arguments");
+    ApiCtx a(tmp, tmpBytes);
+    Ctx& c = a.c;
+    Timer t(c.st);
+    int T = numTriangles;
+    DBuf<int3> adj(T > 0 ? T : 1);
+    DBuf<int> cflag(T > 0 ? T : 1);
+    fill_i32(c, cflag.p, 0, T);
+    build_adjacency(c, (const int3*)*tris, T, numPoints, adj.p);
+    Mesh m{(const int2*)*points, (int3*)*tris, adj.p, nullptr, cflag.p};
+    mark_constraint_chains(c, m, T, numPoints, (const int2*)cons, numCons);
+    int r = 0;
+    flip_rounds(c, m, T, nullptr, 0, &r);
+    if (ms) *ms = t.ms();
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_add_constraints(void* tmp, size_t tmpBytes, const gcdt_int2* points, gcdt_int3* tris, gcdt_int3* adjTris,
+                         int* sons, const int* bitmapOffsets, const int* bitmapTris, const gcdt_int2* cons,
+                         gcdt_int3** newTris, int numCons, int numPoints, int numTriangles, int* numNewTris, double* ms)
+{
+    API_BEGIN
+    (void)bitmapOffsets;
+    (void)bitmapTris;
+    if (!points || !tris || !adjTris || !sons || !newTris || !numNewTris || numCons < 0 || numTriangles < 0 ||
+        (numCons > 0 && !cons))
+        return fail(GCDT_ERR_INVALID, "bad arguments");
+    ApiCtx a(tmp, tmpBytes);
+    Ctx& c = a.c;
+    Timer t(c.st);
+    int Tall = numTriangles;
+    // compact the leaves (exact adjacency) and remember their slots
+    DBuf<int> off((size_t)Tall + 1);
+    k_flags_leaf<<<grid_for(Tall), kBlock, 0, c.st>>>(sons, Tall, off.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl_total(c, off.p, off.p, Tall);
+    int T = c.get(C_TOTAL);
+    DBuf<int3> tri(T > 0 ? T : 1), adj(T > 0 ? T : 1);
+    DBuf<int> slot(T > 0 ? T : 1), cflag(T > 0 ? T : 1), touched(T > 0 ? T : 1);
+    filter_leaves(c, (const int3*)tris, (const int3*)adjTris, sons, Tall, tri.p, adj.p, T);
+    k_leaf_slots<<<grid_for(Tall), kBlock, 0, c.st>>>(sons, off.p, Tall, slot.p);
+    GCDT_LAUNCH_CHECK();
+    fill_i32(c, cflag.p, 0, T);
+    fill_i32(c, touched.p, 0, T);
+    Mesh m{(const int2*)points, tri.p, adj.p, nullptr, cflag.p};
+    insert_constraints(c, m, T, numPoints + 4, (const int2*)cons, numCons, touched.p, nullptr);
+    // crossed triangles are retired in the DAG; their replacements go out
+    k_retire<<<grid_for(T), kBlock, 0, c.st>>>(touched.p, slot.p, T, sons);
+    GCDT_LAUNCH_CHECK();
+    DBuf<int> toff((size_t)T + 1);
+    GCDT_HIP(hipMemcpyAsync(toff.p, touched.p, sizeof(int) * (size_t)T, hipMemcpyDeviceToDevice, c.st));
+    scan_excl_total(c, toff.p, toff.p, T);
+    int nn = c.get(C_TOTAL);
+    DBuf<int3> out(nn > 0 ? nn : 1);
+    k_gather_touched<<<grid_for(T), kBlock, 0, c.st>>>(tri.p, touched.p, toff.p, T, out.p);
+    GCDT_LAUNCH_CHECK();
+    *numNewTris = nn;
+    *newTris = (gcdt_int3*)out.detach();
+    if (ms) *ms = t.ms();
+    return GCDT_OK;
+    API_END
+}
+
+}  // extern "C"

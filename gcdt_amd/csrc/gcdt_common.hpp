@@ -1,0 +1,118 @@
+// Shared definitions for the gfx950 CDT kernels: status codes, error checks,
+// launch geometry and wave64 helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/gcdt.h"
+
+namespace gcdt {
+
+// hipError_t -> exception carrying a GCDT status; caught at the C-ABI edge.
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define GCDT_HIP(expr)                                                                   \
+    do {                                                                                 \
+        hipError_t e__ = (expr);                                                         \
+        if (e__ != hipSuccess) {                                                         \
+            throw ::gcdt::Error(e__ == hipErrorOutOfMemory ? GCDT_ERR_OOM : GCDT_ERR_DEVICE, \
+                                std::string(#expr) + ": " + hipGetErrorString(e__));     \
+        }                                                                                \
+    } while (0)
+
+#define GCDT_LAUNCH_CHECK() GCDT_HIP(hipGetLastError())
+
+constexpr int kBlock = 256;         // 4 wave64 per workgroup
+constexpr int kMaxGrid = 256 * 16;  // persistent grid-stride cap: 16 WGs per CU
+
+inline int grid_for(long long n)
+{
+    long long g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > kMaxGrid) g = kMaxGrid;
+    return (int)g;
+}
+
+// Wave-uniform grid-stride loop over [0, n): every lane of a wave stays in the
+// loop for the same iterations (lanes past the end see valid == false), so
+// wave-level ballots and shuffles inside the body are always converged.
+#define WAVE_LOOP(i, n)                                                                    \
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x,                   \
+                   i##_wb = i - (long long)__lane_id();                                    \
+         i##_wb < (long long)(n);                                                          \
+         i += (long long)gridDim.x * blockDim.x, i##_wb += (long long)gridDim.x * blockDim.x)
+
+// Exclusive prefix sum of v across the 64 lanes; all lanes must be active.
+__device__ inline int wave_excl_scan(int v, int& total)
+{
+    int lane = __lane_id();
+    int incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    total = __shfl(incl, 63, 64);
+    return incl - v;
+}
+
+// Reserve k consecutive entries of a device-wide counter per lane with one
+// atomic per wave; all lanes must be active (k = 0 for lanes that need none).
+__device__ inline int wave_alloc(int k, int* counter)
+{
+    int total;
+    int ex = wave_excl_scan(k, total);
+    int base = 0;
+    if (__lane_id() == 0 && total > 0) base = atomicAdd(counter, total);
+    base = __shfl(base, 0, 64);
+    return base + ex;
+}
+
+__device__ __host__ inline int nx3(int k) { return k == 2 ? 0 : k + 1; }
+__device__ __host__ inline int pv3(int k) { return k == 0 ? 2 : k - 1; }
+
+__device__ __host__ inline int get3(const int3& t, int k) { return k == 0 ? t.x : (k == 1 ? t.y : t.z); }
+__device__ __host__ inline void set3(int3& t, int k, int v)
+{
+    if (k == 0) t.x = v;
+    else if (k == 1) t.y = v;
+    else t.z = v;
+}
+// slot of vertex v in t, or -1
+__device__ __host__ inline int slot_of(const int3& t, int v) { return t.x == v ? 0 : (t.y == v ? 1 : (t.z == v ? 2 : -1)); }
+// slot of the vertex of t that is neither u nor v
+__device__ __host__ inline int slot_other(const int3& t, int u, int v)
+{
+    if (t.x != u && t.x != v) return 0;
+    if (t.y != u && t.y != v) return 1;
+    return 2;
+}
+__device__ __host__ inline bool has2(const int3& t, int u, int v)
+{
+    return (t.x == u || t.y == u || t.z == u) && (t.x == v || t.y == v || t.z == v);
+}
+
+// Adjacency entries written by a remeshing step before the outer neighbour is
+// known to be final: -(old_neighbour + 2).  -1 stays "no neighbour".
+__device__ __host__ inline int adj_mark(int m) { return m < 0 ? -1 : -(m + 2); }
+__device__ __host__ inline bool adj_is_mark(int a) { return a <= -2; }
+__device__ __host__ inline int adj_unmark(int a) { return -a - 2; }
+
+__device__ __host__ inline uint32_t hash32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+}  // namespace gcdt

@@ -1,0 +1,989 @@
+// gfx950 kernels and host drivers of the mesh phases: scans, sampling,
+// bitmap ("boostmap") location, parallel insertion rounds, flip rounds,
+// leaf compaction and adjacency construction.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mesh.hpp"
+
+namespace gcdt {
+
+// ================================================================== context
+
+Ctx::Ctx(hipStream_t s) : st(s)
+{
+    dcnt.alloc(C_NUM);
+    GCDT_HIP(hipHostMalloc((void**)&hcnt, sizeof(int) * C_NUM, hipHostMallocDefault));
+    zero_counters();
+}
+
+Ctx::~Ctx()
+{
+    if (hcnt) (void)hipHostFree(hcnt);
+}
+
+void Ctx::read(int i, int k)
+{
+    GCDT_HIP(hipMemcpyAsync(hcnt + i, dcnt.p + i, sizeof(int) * k, hipMemcpyDeviceToHost, st));
+    GCDT_HIP(hipStreamSynchronize(st));
+}
+
+__global__ void k_set1(int* p, int v) { *p = v; }
+
+void Ctx::set(int i, int v)
+{
+    k_set1<<<1, 1, 0, st>>>(dcnt.p + i, v);
+    GCDT_LAUNCH_CHECK();
+}
+
+void Ctx::zero_counters() { fill_i32(*this, dcnt.p, 0, C_NUM); }
+
+void Ctx::check_err()
+{
+    int e = get(C_ERR);
+    if (e == 0) return;
+    int status = GCDT_ERR_INTERNAL;
+    if (e & MERR_CAPACITY) status = GCDT_ERR_CAPACITY;
+    else if (e & (MERR_LOCATE | MERR_DUPLICATE)) status = GCDT_ERR_LOCATE;
+    else if (e & MERR_CROSSING) status = GCDT_ERR_CROSSING;
+    else if (e & MERR_SCRATCH) status = GCDT_ERR_SCRATCH;
+    else if (e & MERR_INVALID) status = GCDT_ERR_INVALID;
+    throw Error(status, "device consistency flag " + std::to_string(e));
+}
+
+// ================================================================== live kernel timing
+
+namespace {
+struct ProfRec {
+    std::string name;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    long long launches = 0;
+    long long items = 0;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+}  // namespace
+
+ProfScope::ProfScope(const char* name, hipStream_t s, long long items) : st(s)
+{
+    if (!g_prof_on) return;
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (size_t i = 0; i < g_prof.size(); i++)
+        if (g_prof[i].name == name) slot = (int)i;
+    if (slot < 0) {
+        g_prof.push_back(ProfRec{name, {}, 0, 0});
+        slot = (int)g_prof.size() - 1;
+    }
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        slot = -1;
+        return;
+    }
+    (void)hipEventRecord(a, st);
+    g_prof[slot].ev.push_back({a, b});
+    g_prof[slot].launches++;
+    g_prof[slot].items += items;
+}
+
+ProfScope::~ProfScope()
+{
+    if (slot < 0) return;
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    (void)hipEventRecord(g_prof[slot].ev.back().second, st);
+}
+
+void prof_enable(bool on)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof_on = on;
+}
+
+void prof_reset()
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (auto& r : g_prof)
+        for (auto& e : r.ev) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+    g_prof.clear();
+}
+
+// total ms, launches and items of one kernel (synchronises its events)
+bool prof_read(const char* name, double* ms, long long* launches, long long* items)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (auto& r : g_prof) {
+        if (r.name != name) continue;
+        double t = 0;
+        for (auto& e : r.ev) {
+            (void)hipEventSynchronize(e.second);
+            float f = 0;
+            (void)hipEventElapsedTime(&f, e.first, e.second);
+            t += f;
+        }
+        *ms = t;
+        *launches = r.launches;
+        *items = r.items;
+        return true;
+    }
+    return false;
+}
+
+int prof_names(char* buf, int cap)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    std::string all;
+    for (auto& r : g_prof) {
+        all += r.name;
+        all += ';';
+    }
+    if (buf && cap > 0) {
+        int n = (int)std::min<size_t>(all.size(), (size_t)cap - 1);
+        memcpy(buf, all.data(), n);
+        buf[n] = 0;
+    }
+    return (int)all.size();
+}
+
+// ================================================================== primitives
+
+__global__ __launch_bounds__(kBlock) void k_fill_i32(int* p, int v, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u64(unsigned long long* p, unsigned long long v, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void fill_i32(Ctx& c, int* p, int v, long long n)
+{
+    if (n <= 0) return;
+    k_fill_i32<<<grid_for(n), kBlock, 0, c.st>>>(p, v, n);
+    GCDT_LAUNCH_CHECK();
+}
+
+void fill_u64(Ctx& c, unsigned long long* p, unsigned long long v, long long n)
+{
+    if (n <= 0) return;
+    k_fill_u64<<<grid_for(n), kBlock, 0, c.st>>>(p, v, n);
+    GCDT_LAUNCH_CHECK();
+}
+
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = kBlock * SCAN_ITEMS;
+constexpr int SCAN_PAD(int i) { return i + (i >> 4); }
+
+__device__ inline int block_excl_scan(int v, int& total)
+{
+    __shared__ int wsum[kBlock / 64];
+    int lane = __lane_id(), w = threadIdx.x >> 6;
+    int wt;
+    int ex = wave_excl_scan(v, wt);
+    if (lane == 0) wsum[w] = wt;
+    __syncthreads();
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; i++) {
+        int s = wsum[i];
+        if (i < w) base += s;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return base + ex;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tile_sums(const int* in, long long n, int* sums)
+{
+    long long base = (long long)blockIdx.x * SCAN_TILE;
+    int s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        long long i = base + (long long)j * kBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    int tot;
+    block_excl_scan(s, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_sums(int* sums, int nt, int* total, int* total2)
+{
+    __shared__ int wsum[16];
+    __shared__ int carry_s;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (int base = 0; base < nt; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < nt ? sums[i] : 0;
+        int lane = __lane_id(), w = threadIdx.x >> 6;
+        int wt;
+        int ex = wave_excl_scan(v, wt);
+        if (lane == 0) wsum[w] = wt;
+        __syncthreads();
+        int pre = 0, tot = 0;
+        for (int q = 0; q < 16; q++) {
+            if (q < w) pre += wsum[q];
+            tot += wsum[q];
+        }
+        int carry = carry_s;
+        if (i < nt) sums[i] = carry + pre + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry_s = carry + tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (total) *total = carry_s;
+        if (total2) *total2 = carry_s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int* out, long long n, const int* sums)
+{
+    __shared__ int tile[SCAN_PAD(SCAN_TILE)];
+    long long base = (long long)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        int li = j * kBlock + threadIdx.x;
+        long long i = base + li;
+        tile[SCAN_PAD(li)] = i < n ? in[i] : 0;
+    }
+    __syncthreads();
+    int v[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        v[j] = tile[SCAN_PAD(threadIdx.x * SCAN_ITEMS + j)];
+        s += v[j];
+    }
+    int tot;
+    int run = sums[blockIdx.x] + block_excl_scan(s, tot);
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        tile[SCAN_PAD(threadIdx.x * SCAN_ITEMS + j)] = run;
+        run += v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        int li = j * kBlock + threadIdx.x;
+        long long i = base + li;
+        if (i < n) out[i] = tile[SCAN_PAD(li)];
+    }
+}
+
+long long scan_scratch_ints(long long n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+// Exclusive scan of in[0..n) into out[0..n); the total goes to dcnt[C_TOTAL]
+// and, when total_at_end, also to out[n] (out must then hold n+1 ints).
+static void scan_impl(Ctx& c, const int* in, int* out, long long n, bool total_at_end)
+{
+    if (n <= 0) {
+        c.set(C_TOTAL, 0);
+        if (total_at_end) {
+            k_set1<<<1, 1, 0, c.st>>>(out, 0);
+            GCDT_LAUNCH_CHECK();
+        }
+        return;
+    }
+    long long nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    int* sums;
+    if (c.ext_scratch) {
+        if (c.ext_scratch_bytes < sizeof(int) * (size_t)(nt + 1))
+            throw Error(GCDT_ERR_SCRATCH, "caller scratch too small for a scan of " + std::to_string(n));
+        sums = c.ext_scratch;
+    } else {
+        c.scan_sums.ensure((size_t)nt + 1);
+        sums = c.scan_sums.p;
+    }
+    k_scan_tile_sums<<<(unsigned)nt, kBlock, 0, c.st>>>(in, n, sums);
+    GCDT_LAUNCH_CHECK();
+    k_scan_sums<<<1, 1024, 0, c.st>>>(sums, (int)nt, c.dcnt.p + C_TOTAL, total_at_end ? out + n : nullptr);
+    GCDT_LAUNCH_CHECK();
+    k_scan_apply<<<(unsigned)nt, kBlock, 0, c.st>>>(in, out, n, sums);
+    GCDT_LAUNCH_CHECK();
+}
+
+void scan_excl(Ctx& c, const int* in, int* out, long long n) { scan_impl(c, in, out, n, false); }
+void scan_excl_total(Ctx& c, const int* in, int* out, long long n) { scan_impl(c, in, out, n, true); }
+
+// ================================================================== square / sample
+
+constexpr int GRID_R = (1 << 30) - 1;
+
+__global__ void k_init_square(int2* pts, int3* tri, int3* adj, int n)
+{
+    if (pts) {
+        pts[n] = make_int2(0, 0);
+        pts[n + 1] = make_int2(GRID_R, 0);
+        pts[n + 2] = make_int2(GRID_R, GRID_R);
+        pts[n + 3] = make_int2(0, GRID_R);
+    }
+    // rawMesh.cu:16-21: (N,N+1,N+3) and (N+1,N+2,N+3) split along the anti-diagonal
+    if (tri) {
+        tri[0] = make_int3(n, n + 1, n + 3);
+        tri[1] = make_int3(n + 1, n + 2, n + 3);
+    }
+    if (adj) {
+        adj[0] = make_int3(1, -1, -1);
+        adj[1] = make_int3(-1, 0, -1);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_points(const int2* pts, int n, int* err)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        int2 p = pts[i];
+        if (p.x <= 0 || p.y <= 0 || p.x >= GRID_R || p.y >= GRID_R) atomicOr(err, MERR_INVALID);
+    }
+}
+
+void check_points(Ctx& c, const int2* pts, int n)
+{
+    if (n <= 0) return;
+    k_check_points<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, c.dcnt.p + C_ERR);
+    GCDT_LAUNCH_CHECK();
+    c.check_err();
+}
+
+void init_square(Ctx& c, int2* pts, int3* tri, int3* adj, int n)
+{
+    k_init_square<<<1, 1, 0, c.st>>>(pts, tri, adj, n);
+    GCDT_LAUNCH_CHECK();
+}
+
+__global__ void k_write_corners(int2* pts, int n)
+{
+    pts[n] = make_int2(0, 0);
+    pts[n + 1] = make_int2(GRID_R, 0);
+    pts[n + 2] = make_int2(GRID_R, GRID_R);
+    pts[n + 3] = make_int2(0, GRID_R);
+}
+
+void write_corners(Ctx& c, int2* pts, int n)
+{
+    k_write_corners<<<1, 1, 0, c.st>>>(pts, n);
+    GCDT_LAUNCH_CHECK();
+}
+
+constexpr int LOG_SAMPLE = 8;   // math_base.cuh:24 logSample
+constexpr int LOG_BITMAP = 10;  // math_base.cuh:25 logBitmap
+constexpr int SAMPLE_CELLS = 1 << (2 * LOG_SAMPLE);
+constexpr int BITMAP_CELLS = 1 << (2 * LOG_BITMAP);
+
+__global__ __launch_bounds__(kBlock) void k_sample_cells(const int2* pts, int n, int* cellmin)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        int2 p = pts[i];
+        int cell = ((p.x >> (30 - LOG_SAMPLE)) << LOG_SAMPLE) + (p.y >> (30 - LOG_SAMPLE));
+        atomicMin(&cellmin[cell], (int)i);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sample_flags(const int* cellmin, int* flags)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < SAMPLE_CELLS; i += gridDim.x * blockDim.x)
+        flags[i] = cellmin[i] != INT_MAX;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sample_emit(const int2* pts, const int* cellmin, const int* offs, int* ids,
+                                                        int2* spts)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < SAMPLE_CELLS; i += gridDim.x * blockDim.x) {
+        int p = cellmin[i];
+        if (p != INT_MAX) {
+            int o = offs[i];
+            if (ids) ids[o] = p;
+            if (spts) spts[o] = pts[p];
+        }
+    }
+}
+
+int sample_points(Ctx& c, const int2* pts, int n, int* sample_ids, int2* sample_pts)
+{
+    DBuf<int> cellmin(SAMPLE_CELLS), offs(SAMPLE_CELLS + 1);
+    fill_i32(c, cellmin.p, INT_MAX, SAMPLE_CELLS);
+    if (n > 0) {
+        k_sample_cells<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cellmin.p);
+        GCDT_LAUNCH_CHECK();
+    }
+    k_sample_flags<<<grid_for(SAMPLE_CELLS), kBlock, 0, c.st>>>(cellmin.p, offs.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl(c, offs.p, offs.p, SAMPLE_CELLS);
+    k_sample_emit<<<grid_for(SAMPLE_CELLS), kBlock, 0, c.st>>>(pts, cellmin.p, offs.p, sample_ids, sample_pts);
+    GCDT_LAUNCH_CHECK();
+    return c.get(C_TOTAL);
+}
+
+// ================================================================== location
+
+// initTriangle, rawMesh.cu:37-57: side of the anti-diagonal (R,0)-(0,R)
+__global__ __launch_bounds__(kBlock) void k_locate_two(const int2* pts, int n, int* loc, int* pend)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        int o = orient2d(make_int2(GRID_R, 0), make_int2(0, GRID_R), pts[i]);
+        loc[i] = o > 0 ? 3 : (o < 0 ? 4 + 3 : 0);  // tri 0 inside / tri 1 inside / edge opposite vertex 0 of tri 0
+        pend[i] = (int)i;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_locate_two_ids(const int2* pts, const int* ids, int n, int* loc, int* pend)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        int p = ids[i];
+        int o = orient2d(make_int2(GRID_R, 0), make_int2(0, GRID_R), pts[p]);
+        loc[p] = o > 0 ? 3 : (o < 0 ? 4 + 3 : 0);
+        pend[i] = p;
+    }
+}
+
+int locate_two_ids(Ctx& c, const int2* pts, const int* ids, int n, int* loc, int* pend)
+{
+    if (n > 0) {
+        k_locate_two_ids<<<grid_for(n), kBlock, 0, c.st>>>(pts, ids, n, loc, pend);
+        GCDT_LAUNCH_CHECK();
+    }
+    return n;
+}
+
+int locate_two(Ctx& c, const int2* pts, int n, int* loc, int* pend)
+{
+    if (n > 0) {
+        k_locate_two<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, loc, pend);
+        GCDT_LAUNCH_CHECK();
+    }
+    return n;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mark_vertices(const int3* tri, int T, int n, int* isv)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        int3 t = tri[i];
+        if (t.x < n) isv[t.x] = 1;
+        if (t.y < n) isv[t.y] = 1;
+        if (t.z < n) isv[t.z] = 1;
+    }
+}
+
+// initTriangleBoost, rawMesh.cu:59-152: scan the triangles listed for the
+// point's bitmap cell; the first whose closed region holds the point wins.
+__global__ __launch_bounds__(kBlock) void k_locate_bitmap(const int2* pts, int n, const int* isv, const int3* tri,
+                                                          const int* bm_off, const int* bm_tri, int* loc, int* pend,
+                                                          int* npend, int* err)
+{
+    WAVE_LOOP(i, n)
+    {
+        bool want = false;
+        if (i < n) {
+            int p = (int)i;
+            if (isv[p]) {
+                loc[p] = -1;
+            } else {
+                int2 P = pts[p];
+                int cell = ((P.x >> (30 - LOG_BITMAP)) << LOG_BITMAP) + (P.y >> (30 - LOG_BITMAP));
+                int r = -1, t = -1;
+                for (int j = bm_off[cell], e = bm_off[cell + 1]; j < e; j++) {
+                    t = bm_tri[j];
+                    int3 tv = tri[t];
+                    r = classify(pts[tv.x], pts[tv.y], pts[tv.z], P);
+                    if (r >= 0 || r == -2) break;
+                }
+                if (r >= 0) {
+                    loc[p] = t * 4 + r;
+                    want = true;
+                } else {
+                    loc[p] = -1;
+                    atomicOr(err, r == -2 ? MERR_DUPLICATE : MERR_LOCATE);
+                }
+            }
+        }
+        int slot = wave_alloc(want ? 1 : 0, npend);
+        if (want) pend[slot] = (int)i;
+    }
+}
+
+int locate_bitmap(Ctx& c, const int2* pts, int n, const int3* tri, int T, const int* bm_off, const int* bm_tri,
+                  int* loc, int* isv, int* pend)
+{
+    k_mark_vertices<<<grid_for(T), kBlock, 0, c.st>>>(tri, T, n, isv);
+    GCDT_LAUNCH_CHECK();
+    c.set(C_NPEND, 0);
+    if (n > 0) {
+        ProfScope ps("locate_bitmap", c.st, n);
+        k_locate_bitmap<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, isv, tri, bm_off, bm_tri, loc, pend,
+                                                          c.dcnt.p + C_NPEND, c.dcnt.p + C_ERR);
+        GCDT_LAUNCH_CHECK();
+    }
+    c.check_err();
+    return c.get(C_NPEND);
+}
+
+// ================================================================== insertion rounds
+
+__global__ void k_round_begin(int* cnt, int clear_idx)
+{
+    cnt[C_TBASE] = cnt[C_NTRI];
+    cnt[clear_idx] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ins_claim(Ins s, const int* list, const int* cnt, unsigned epoch)
+{
+    int n = *cnt;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        ins_claim(s, list[i], epoch);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ins_split(Ins s, const int* list, const int* cnt, unsigned epoch,
+                                                      int* ntri)
+{
+    int n = *cnt;
+    WAVE_LOOP(i, n)
+    {
+        int p = i < n ? list[i] : -1;
+        int need = p >= 0 ? ins_need(s, p, epoch) : 0;
+        int base = wave_alloc(need, ntri);
+        if (need) {
+            if (base + need > s.cap) atomicOr(s.err, MERR_CAPACITY);
+            else ins_split(s, p, base);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ins_fix(Ins s, const int* cnt)
+{
+    int b = cnt[C_TBASE], e = min(cnt[C_NTRI], s.cap);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x + b; i < e; i += (long long)gridDim.x * blockDim.x)
+        ins_fix(s, (int)i);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ins_reloc(Ins s, const int* list, const int* cnt, int* list2, int* cnt2)
+{
+    int n = *cnt;
+    WAVE_LOOP(i, n)
+    {
+        int p = i < n ? list[i] : -1;
+        int keep = p >= 0 ? ins_relocate(s, p) : 0;
+        int slot = wave_alloc(keep, cnt2);
+        if (keep) list2[slot] = p;
+    }
+}
+
+int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int npend, int ntri, int* rounds)
+{
+    c.set(C_NPEND, npend);
+    c.set(C_NTRI, ntri);
+    int* P = pend.p;
+    int* P2 = pend2.p;
+    int cp = C_NPEND, cp2 = C_NPEND2;
+    int r = 0;
+    while (npend > 0) {
+        if (++r > 100000) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion rounds did not converge");
+        unsigned ep = c.next_epoch();
+        k_round_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, cp2);
+        GCDT_LAUNCH_CHECK();
+        int g = grid_for(npend);
+        {
+            ProfScope ps("ins_claim", c.st, npend);
+            k_ins_claim<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, ep);
+        }
+        GCDT_LAUNCH_CHECK();
+        {
+            ProfScope ps("ins_split", c.st, npend);
+            k_ins_split<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, ep, c.dcnt.p + C_NTRI);
+        }
+        GCDT_LAUNCH_CHECK();
+        {
+            ProfScope ps("ins_fix", c.st, 0);
+            k_ins_fix<<<grid_for(4LL * npend), kBlock, 0, c.st>>>(s, c.dcnt.p);
+        }
+        GCDT_LAUNCH_CHECK();
+        {
+            ProfScope ps("ins_reloc", c.st, npend);
+            k_ins_reloc<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, P2, c.dcnt.p + cp2);
+        }
+        GCDT_LAUNCH_CHECK();
+        std::swap(P, P2);
+        std::swap(cp, cp2);
+        c.read(0, 8);
+        if (c.hcnt[C_ERR]) c.check_err();
+        // every round inserts at least the globally highest-priority point
+        if (c.hcnt[cp] >= npend && c.hcnt[C_NTRI] == c.hcnt[C_TBASE])
+            throw Error(GCDT_ERR_INTERNAL, "insertion round without progress");
+        npend = c.hcnt[cp];
+    }
+    if (rounds) *rounds = r;
+    int nt = c.get(C_NTRI);
+    if (nt > s.cap) throw Error(GCDT_ERR_CAPACITY, "triangle capacity exceeded");
+    return nt;
+}
+
+// ================================================================== flip rounds
+
+__global__ void k_flip_begin(int* cnt)
+{
+    cnt[C_NACT2] = 0;
+    cnt[C_NFLIP] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_flip_test(Flip f, const int* list, const int* cnt, unsigned epoch)
+{
+    int n = *cnt;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        flip_test(f, list ? list[i] : (int)i, epoch);
+}
+
+__global__ __launch_bounds__(kBlock) void k_flip_apply(Flip f, const int* list, const int* cnt, unsigned epoch,
+                                                       int* qround, int* list2, int* cnt2, int* flipped, int* nflip)
+{
+    int n = *cnt;
+    WAVE_LOOP(i, n)
+    {
+        int t = -1, nn = -1, res = 0;
+        if (i < n) {
+            t = list ? list[i] : (int)i;
+            res = flip_apply(f, t, epoch, &nn);
+        }
+        int q0 = -1, q1 = -1;
+        if (res >= 1 && atomicMax(&qround[t], (int)epoch) < (int)epoch) q0 = t;
+        if (res == 2 && atomicMax(&qround[nn], (int)epoch) < (int)epoch) q1 = nn;
+        int k = (q0 >= 0) + (q1 >= 0);
+        int b = wave_alloc(k, cnt2);
+        if (q0 >= 0) list2[b++] = q0;
+        if (q1 >= 0) list2[b] = q1;
+        int fb = wave_alloc(res == 2 ? 2 : 0, nflip);
+        if (res == 2) {
+            flipped[fb] = t;
+            flipped[fb + 1] = nn;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_flip_fix(Flip f, const int* list, const int* cnt, unsigned epoch)
+{
+    int n = *cnt;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        flip_fix(f, list[i], epoch);
+}
+
+long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds)
+{
+    DBuf<unsigned long long> claim(T);
+    DBuf<int> stamp(T), partner(T), cand(T), qround(T), la(T), lb(T), lf(T);
+    fill_u64(c, claim.p, 0, T);
+    fill_i32(c, stamp.p, 0, T);
+    fill_i32(c, qround.p, 0, T);
+    Flip f;
+    f.m = m;
+    f.claim = claim.p;
+    f.stamp = stamp.p;
+    f.partner = partner.p;
+    f.cand = cand.p;
+    f.err = c.dcnt.p + C_ERR;
+    int nact = init ? ninit : T;
+    const int* act = init;
+    int* nxt = la.p;
+    int* other = lb.p;
+    // implicit list: the count lives in C_CONST; explicit: in C_NACT
+    if (init) {
+        c.set(C_NACT, ninit);
+    } else {
+        c.set(C_CONST, T);
+    }
+    int cnt_idx = init ? C_NACT : C_CONST;
+    long long flips = 0;
+    int r = 0;
+    while (nact > 0) {
+        if (++r > 1000000) throw Error(GCDT_ERR_NOT_CONVERGED, "flip rounds did not converge");
+        unsigned ep = c.next_epoch();
+        k_flip_begin<<<1, 1, 0, c.st>>>(c.dcnt.p);
+        GCDT_LAUNCH_CHECK();
+        int g = grid_for(nact);
+        {
+            ProfScope ps("flip_test", c.st, nact);
+            k_flip_test<<<g, kBlock, 0, c.st>>>(f, act, c.dcnt.p + cnt_idx, ep);
+        }
+        GCDT_LAUNCH_CHECK();
+        {
+            ProfScope ps("flip_apply", c.st, nact);
+            k_flip_apply<<<g, kBlock, 0, c.st>>>(f, act, c.dcnt.p + cnt_idx, ep, qround.p, nxt, c.dcnt.p + C_NACT2,
+                                                 lf.p, c.dcnt.p + C_NFLIP);
+        }
+        GCDT_LAUNCH_CHECK();
+        {
+            ProfScope ps("flip_fix", c.st, 0);
+            k_flip_fix<<<grid_for(nact), kBlock, 0, c.st>>>(f, lf.p, c.dcnt.p + C_NFLIP, ep);
+        }
+        GCDT_LAUNCH_CHECK();
+        c.read(0, 8);
+        if (c.hcnt[C_ERR]) c.check_err();
+        // the highest-keyed proposal always wins both of its triangles
+        if (c.hcnt[C_NACT2] > 0 && c.hcnt[C_NFLIP] == 0) throw Error(GCDT_ERR_INTERNAL, "flip round without progress");
+        nact = c.hcnt[C_NACT2];
+        flips += c.hcnt[C_NFLIP] / 2;
+        c.set(C_NACT, nact);
+        act = nxt;
+        cnt_idx = C_NACT;
+        std::swap(nxt, other);
+        if (c.hcnt[C_ERR]) c.check_err();
+    }
+    if (rounds) *rounds = r;
+    return flips;
+}
+
+// ================================================================== leaves
+
+__global__ __launch_bounds__(kBlock) void k_leaf_flags(const int* sons, int T, int* flags)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x)
+        flags[i] = sons[i] == -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaf_scatter(const int3* tri, const int3* adj, const int* sons, const int* off,
+                                                         int T, int3* tri2, int3* adj2, long long cap, int* err)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        if (sons[i] != -1) continue;
+        int o = off[i];
+        if (o >= cap) {
+            atomicOr(err, MERR_CAPACITY);
+            continue;
+        }
+        tri2[o] = tri[i];
+        if (adj && adj2) {
+            int3 a = adj[i];
+            a.x = a.x >= 0 ? off[a.x] : -1;
+            a.y = a.y >= 0 ? off[a.y] : -1;
+            a.z = a.z >= 0 ? off[a.z] : -1;
+            adj2[o] = a;
+        }
+    }
+}
+
+int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2, int3* adj2,
+                  long long cap)
+{
+    DBuf<int> off((size_t)T + 1);
+    k_leaf_flags<<<grid_for(T), kBlock, 0, c.st>>>(sons, T, off.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl(c, off.p, off.p, T);
+    k_leaf_scatter<<<grid_for(T), kBlock, 0, c.st>>>(tri, adj, sons, off.p, T, tri2, adj2, cap, c.dcnt.p + C_ERR);
+    GCDT_LAUNCH_CHECK();
+    int n = c.get(C_TOTAL);
+    c.check_err();
+    return n;
+}
+
+// ================================================================== bitmap
+
+__device__ inline void tri_cells(int2 A, int2 B, int2 C, int& x0, int& y0, int& x1, int& y1)
+{
+    constexpr int sh = 30 - LOG_BITMAP;
+    x0 = min(A.x, min(B.x, C.x)) >> sh;
+    x1 = max(A.x, max(B.x, C.x)) >> sh;
+    y0 = min(A.y, min(B.y, C.y)) >> sh;
+    y1 = max(A.y, max(B.y, C.y)) >> sh;
+}
+
+// Closed triangle vs closed box, exact (separating-axis test: the box axes,
+// then each triangle edge with all four corners strictly outside).
+// Same predicate as the reference's intersect(), boostmap.cu:30-66.
+__device__ inline bool tri_meets_box(int2 A, int2 B, int2 C, int bx0, int by0, int bx1, int by1)
+{
+    if (max(A.x, max(B.x, C.x)) < bx0 || min(A.x, min(B.x, C.x)) > bx1) return false;
+    if (max(A.y, max(B.y, C.y)) < by0 || min(A.y, min(B.y, C.y)) > by1) return false;
+    int2 q[4] = {make_int2(bx0, by0), make_int2(bx1, by0), make_int2(bx1, by1), make_int2(bx0, by1)};
+    int2 P[3] = {A, B, C};
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        int2 u = P[e], v = P[(e + 1) % 3];
+        bool out = true;
+#pragma unroll
+        for (int j = 0; j < 4; j++) out &= orient2d(u, v, q[j]) < 0;
+        if (out) return false;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bm_count(const int2* pts, const int3* tri, int T, int* cnt)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        int3 t = tri[i];
+        int x0, y0, x1, y1;
+        tri_cells(pts[t.x], pts[t.y], pts[t.z], x0, y0, x1, y1);
+        cnt[i] = (x1 - x0 + 1) * (y1 - y0 + 1);
+    }
+}
+
+__device__ inline int upper_bound_i(const int* a, int n, long long x)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_bm_test(const int2* pts, const int3* tri, const int* off, int T,
+                                                    long long npairs, int* cellcnt, int* ord, int* cellof)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += (long long)gridDim.x * blockDim.x) {
+        int t = upper_bound_i(off, T + 1, i) - 1;
+        int local = (int)(i - off[t]);
+        int3 tv = tri[t];
+        int2 A = pts[tv.x], B = pts[tv.y], C = pts[tv.z];
+        int x0, y0, x1, y1;
+        tri_cells(A, B, C, x0, y0, x1, y1);
+        int h = y1 - y0 + 1;
+        int cx = x0 + local / h, cy = y0 + local % h;
+        constexpr int sh = 30 - LOG_BITMAP;
+        bool hit = tri_meets_box(A, B, C, cx << sh, cy << sh, ((cx + 1) << sh) - 1, ((cy + 1) << sh) - 1);
+        int cell = (cx << LOG_BITMAP) + cy;
+        cellof[i] = cell;
+        ord[i] = hit ? atomicAdd(&cellcnt[cell], 1) : -1;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bm_fill(const int* off, int T, long long npairs, const int* ord,
+                                                    const int* cellof, const int* cell_off, int* bm)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += (long long)gridDim.x * blockDim.x) {
+        int o = ord[i];
+        if (o < 0) continue;
+        int t = upper_bound_i(off, T + 1, i) - 1;
+        bm[cell_off[cellof[i]] + o] = t;
+    }
+}
+
+void build_bitmap(Ctx& c, const int2* pts, const int3* tri, int T, int** offsets, int** tris, int* ntris)
+{
+    DBuf<int> off((size_t)T + 1);
+    k_bm_count<<<grid_for(T), kBlock, 0, c.st>>>(pts, tri, T, off.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl_total(c, off.p, off.p, T);
+    long long npairs = c.get(C_TOTAL);
+    DBuf<int> ord(npairs), cellof(npairs);
+    DBuf<int> cell_off(BITMAP_CELLS + 1);
+    fill_i32(c, cell_off.p, 0, BITMAP_CELLS + 1);
+    if (npairs > 0) {
+        k_bm_test<<<grid_for(npairs), kBlock, 0, c.st>>>(pts, tri, off.p, T, npairs, cell_off.p, ord.p, cellof.p);
+        GCDT_LAUNCH_CHECK();
+    }
+    scan_excl_total(c, cell_off.p, cell_off.p, BITMAP_CELLS);
+    int total = c.get(C_TOTAL);
+    DBuf<int> bm(total > 0 ? total : 1);
+    if (npairs > 0) {
+        k_bm_fill<<<grid_for(npairs), kBlock, 0, c.st>>>(off.p, T, npairs, ord.p, cellof.p, cell_off.p, bm.p);
+        GCDT_LAUNCH_CHECK();
+    }
+    // hand the buffers to the caller; they are released with the sizes
+    // (2^20+1) ints and `total` ints
+    *offsets = cell_off.detach();
+    *tris = bm.detach();
+    *ntris = total;
+}
+
+// ================================================================== adjacency from a triangle list
+
+__global__ __launch_bounds__(kBlock) void k_adj_count(const int3* tri, int T, int* cnt)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < 3LL * T; i += (long long)gridDim.x * blockDim.x) {
+        int t = (int)(i / 3), k = (int)(i % 3);
+        int3 tv = tri[t];
+        int u = get3(tv, nx3(k)), v = get3(tv, pv3(k));
+        atomicAdd(&cnt[min(u, v)], 1);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_fill(const int3* tri, int T, const int* off, int* cur, int* bucket)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < 3LL * T; i += (long long)gridDim.x * blockDim.x) {
+        int t = (int)(i / 3), k = (int)(i % 3);
+        int3 tv = tri[t];
+        int u = get3(tv, nx3(k)), v = get3(tv, pv3(k));
+        int key = min(u, v);
+        bucket[off[key] + atomicAdd(&cur[key], 1)] = (int)i;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_match(const int3* tri, int T, const int* off, const int* bucket, int* adj)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < 3LL * T; i += (long long)gridDim.x * blockDim.x) {
+        int t = (int)(i / 3), k = (int)(i % 3);
+        int3 tv = tri[t];
+        int u = get3(tv, nx3(k)), v = get3(tv, pv3(k));
+        int key = min(u, v);
+        int r = -1;
+        for (int j = off[key], e = off[key + 1]; j < e; j++) {
+            int h = bucket[j];
+            if (h == (int)i) continue;
+            int3 w = tri[h / 3];
+            int kk = h % 3;
+            if (get3(w, nx3(kk)) == v && get3(w, pv3(kk)) == u) {
+                r = h / 3;
+                break;
+            }
+        }
+        adj[i] = r;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cons_mark(const int3* tri, const int* off, const int* bucket, const int2* cons,
+                                                      int C, int* cflag)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < C; i += (long long)gridDim.x * blockDim.x) {
+        int2 e = cons[i];
+        int key = min(e.x, e.y), other = max(e.x, e.y);
+        for (int j = off[key], f = off[key + 1]; j < f; j++) {
+            int h = bucket[j];
+            int3 w = tri[h / 3];
+            int kk = h % 3;
+            int a = get3(w, nx3(kk)), b = get3(w, pv3(kk));
+            if (max(a, b) == other) atomicOr(&cflag[h / 3], 1 << kk);
+        }
+    }
+}
+
+void build_adjacency(Ctx& c, const int3* tri, int T, int nv, int3* adj)
+{
+    DBuf<int> off((size_t)nv + 1), cur(nv), bucket(3LL * T);
+    fill_i32(c, off.p, 0, nv + 1);
+    fill_i32(c, cur.p, 0, nv);
+    k_adj_count<<<grid_for(3LL * T), kBlock, 0, c.st>>>(tri, T, off.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl_total(c, off.p, off.p, nv);
+    k_adj_fill<<<grid_for(3LL * T), kBlock, 0, c.st>>>(tri, T, off.p, cur.p, bucket.p);
+    GCDT_LAUNCH_CHECK();
+    k_adj_match<<<grid_for(3LL * T), kBlock, 0, c.st>>>(tri, T, off.p, bucket.p, (int*)adj);
+    GCDT_LAUNCH_CHECK();
+}
+
+void mark_constraint_edges(Ctx& c, const int3* tri, const int3* adj, int T, int nv, const int2* cons, int C, int* cflag)
+{
+    (void)adj;
+    DBuf<int> off((size_t)nv + 1), cur(nv), bucket(3LL * T);
+    fill_i32(c, off.p, 0, nv + 1);
+    fill_i32(c, cur.p, 0, nv);
+    k_adj_count<<<grid_for(3LL * T), kBlock, 0, c.st>>>(tri, T, off.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl_total(c, off.p, off.p, nv);
+    k_adj_fill<<<grid_for(3LL * T), kBlock, 0, c.st>>>(tri, T, off.p, cur.p, bucket.p);
+    GCDT_LAUNCH_CHECK();
+    if (C > 0) {
+        k_cons_mark<<<grid_for(C), kBlock, 0, c.st>>>(tri, off.p, bucket.p, cons, C, cflag);
+        GCDT_LAUNCH_CHECK();
+    }
+}
+
+}  // namespace gcdt

@@ -1,0 +1,174 @@
+// Host-side interface of the mesh phases (internal to the library).
+#pragma once
+#include <utility>
+
+#include "../../include/MemCache.h"
+#include "mesh_bodies.hpp"
+
+namespace gcdt {
+
+// Device buffer from the MemCache pool, released on destruction.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    void alloc(size_t count)
+    {
+        release();
+        n = count;
+        p = (T*)getMemCacheRef().getMemByAbtrtSize(sizeof(T) * (count ? count : 1));
+        if (!p) throw Error(GCDT_ERR_OOM, "device pool allocation failed");
+    }
+    void ensure(size_t count)
+    {
+        if (count > n || !p) alloc(count);
+    }
+    void release()
+    {
+        if (p) getMemCacheRef().releaseAbtrtMem(p, sizeof(T) * (n ? n : 1));
+        p = nullptr;
+        n = 0;
+    }
+    T* detach()
+    {
+        T* q = p;
+        p = nullptr;
+        n = 0;
+        return q;
+    }
+    ~DBuf() { release(); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf& operator=(DBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+};
+
+// device counters (indices into Ctx::dcnt)
+enum : int {
+    C_NPEND = 0,
+    C_NPEND2 = 1,
+    C_NTRI = 2,
+    C_TBASE = 3,
+    C_ERR = 4,
+    C_NACT = 5,
+    C_NACT2 = 6,
+    C_NFLIP = 7,
+    C_TOTAL = 8,   // scan totals
+    C_CONST = 9,   // a constant count for implicit lists
+    C_NPIECE = 10,
+    C_NQ = 11,
+    C_NQ2 = 12,
+    C_SCR = 13,    // cavity scratch cursor
+    C_NREW = 14,   // rewritten triangles this round
+    C_NWIN = 15,
+    C_NUM = 32
+};
+
+struct Ctx {
+    hipStream_t st = nullptr;
+    DBuf<int> dcnt;
+    int* hcnt = nullptr;  // pinned mirror
+    unsigned epoch = 0;
+    DBuf<int> scan_sums;
+    // optional caller scratch (the reference's cubTempStorage): scans use it
+    // instead of the pool when set, and fail with GCDT_ERR_SCRATCH if small
+    int* ext_scratch = nullptr;
+    size_t ext_scratch_bytes = 0;
+    explicit Ctx(hipStream_t s);
+    ~Ctx();
+    Ctx(const Ctx&) = delete;
+    Ctx& operator=(const Ctx&) = delete;
+    unsigned next_epoch() { return ++epoch; }
+    // copy counters [i, i+k) to hcnt and wait
+    void read(int i, int k = 1);
+    int get(int i)
+    {
+        read(i, 1);
+        return hcnt[i];
+    }
+    void set(int i, int v);
+    void zero_counters();
+    void check_err();  // throws on a device error flag
+};
+
+// Live per-kernel timing (enabled through the C ABI): launches of the
+// kernels named in prof_kernels are bracketed by hipEvents on their stream;
+// items = list entries the launch processed (for algorithmic bytes).
+struct ProfScope {
+    int slot = -1;
+    hipStream_t st = nullptr;
+    ProfScope(const char* name, hipStream_t s, long long items);
+    ~ProfScope();
+};
+
+void prof_enable(bool on);
+void prof_reset();
+bool prof_read(const char* name, double* ms, long long* launches, long long* items);
+int prof_names(char* buf, int cap);
+
+// primitives
+void fill_i32(Ctx& c, int* p, int v, long long n);
+void fill_u64(Ctx& c, unsigned long long* p, unsigned long long v, long long n);
+// exclusive scan of n ints (out may alias in); total also to dcnt[C_TOTAL]
+void scan_excl(Ctx& c, const int* in, int* out, long long n);
+// same, and out[n] = total (out holds n+1 ints)
+void scan_excl_total(Ctx& c, const int* in, int* out, long long n);
+long long scan_scratch_ints(long long n);
+
+// mesh construction
+void init_square(Ctx& c, int2* pts, int3* tri, int3* adj, int n);
+// flags GCDT_ERR_INVALID unless every point is strictly inside the square
+void check_points(Ctx& c, const int2* pts, int n);
+// sample: one point per occupied 256x256 cell (smallest id); returns count
+int sample_points(Ctx& c, const int2* pts, int n, int* sample_ids, int2* sample_pts);
+// insertion rounds; pend holds npend point ids with loc[] set; tri/adj/sons
+// hold ntri slots. Returns the new slot count.
+int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int npend, int ntri,
+                  int* rounds);
+// flip to (constrained) Delaunay; init list (nullptr = all T triangles)
+long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds);
+// leaves of a DAG -> compact arrays (adjacency remapped); returns count
+int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2, int3* adj2,
+                  long long cap);
+// bitmap CSR over 1024^2 cells; allocates offsets/tris from the pool
+void build_bitmap(Ctx& c, const int2* pts, const int3* tri, int T, int** offsets, int** tris, int* ntris);
+// mark vertices of tri[0..T) in isv (must be zeroed), then locate the other
+// points of [0,n) through the bitmap; fills loc and pend; returns npend
+int locate_bitmap(Ctx& c, const int2* pts, int n, const int3* tri, int T, const int* bm_off, const int* bm_tri,
+                  int* loc, int* isv, int* pend);
+// locate all points of [0,n) in the two corner triangles 0/1
+int locate_two(Ctx& c, const int2* pts, int n, int* loc, int* pend);
+int locate_two_ids(Ctx& c, const int2* pts, const int* ids, int n, int* loc, int* pend);
+// adjacency of a plain triangle list (edges matched through vertex buckets)
+void build_adjacency(Ctx& c, const int3* tri, int T, int nv, int3* adj);
+// constrained flags: bit k of cflag[t] set when edge k of t equals a constraint
+void mark_constraint_edges(Ctx& c, const int3* tri, const int3* adj, int T, int nv, const int2* cons, int C,
+                           int* cflag);
+// constraint recovery by cavity retriangulation; touched (optional, T ints)
+// receives 1 for every triangle slot that was rewritten
+struct ConsResult {
+    int rounds = 0;
+    long long pieces = 0;
+    long long rewritten = 0;
+};
+ConsResult insert_constraints(Ctx& c, const Mesh& m, int T, int nv, const int2* cons, int C, int* touched,
+                              int* ntouched);
+// mark every edge chain of the mesh that runs exactly along a constraint
+// (constraints split at collinear vertices) as constrained; returns pieces
+long long mark_constraint_chains(Ctx& c, const Mesh& m, int T, int nv, const int2* cons, int C);
+// vertex ids n..n+3 <- the square corners (insertCornerPoints)
+void write_corners(Ctx& c, int2* pts, int n);
+
+}  // namespace gcdt

@@ -1,0 +1,327 @@
+// Per-item bodies of the mesh kernels (point insertion, edge flips, mesh
+// bookkeeping).  Each body handles one list item; the __global__ wrappers in
+// mesh.hip add the wave-level list compaction around them.
+//
+// Mesh layout (identical to the reference's rawMesh layout, rawMesh.cu:5-22,
+// 200-280):
+//   pts[v]      int2 on the 2^30 grid; the 4 square corners are ids n..n+3
+//   tri[t]      int3 vertex ids, counter-clockwise
+//   adj[t]      int3, component k = neighbour across the edge opposite vertex k
+//               (-1 = none)
+//   sons[t]     insertion DAG: -1 leaf, >0 first of 3 children
+//               (q,y,z),(q,z,x),(q,x,y) of an interior split by q, <0 minus the
+//               first of 2 children (o,a,q),(o,q,b) of a split of edge (a,b)
+//               of (o,a,b) by q
+//   cflag[t]    bit k set = the edge opposite vertex k is a constraint
+// Point state during insertion, loc[p]: -1 inserted / not pending, otherwise
+// t*4+k with k==3 "strictly inside leaf t" and k<3 "on the edge opposite
+// vertex k of leaf t".
+#pragma once
+#include "gcdt_common.hpp"
+#include "predicates.hpp"
+
+namespace gcdt {
+
+// device atomics used by the bodies
+__device__ inline unsigned long long at_max(unsigned long long* p, unsigned long long v) { return atomicMax(p, v); }
+__device__ inline unsigned long long at_min(unsigned long long* p, unsigned long long v) { return atomicMin(p, v); }
+__device__ inline int at_max(int* p, int v) { return atomicMax(p, v); }
+__device__ inline int at_min(int* p, int v) { return atomicMin(p, v); }
+__device__ inline int at_or(int* p, int v) { return atomicOr(p, v); }
+__device__ inline int at_add(int* p, int v) { return atomicAdd(p, v); }
+
+enum : int {
+    MERR_CAPACITY = 1,     // triangle slots exhausted
+    MERR_LOCATE = 2,       // a point was not found in any candidate triangle
+    MERR_ADJ = 4,          // adjacency could not be resolved
+    MERR_DUPLICATE = 8,    // a point coincides with a vertex
+    MERR_CROSSING = 16,    // two constraints cross
+    MERR_WALK = 32,        // a constraint walk left the mesh
+    MERR_CAVITY = 64,      // cavity retriangulation inconsistency
+    MERR_SCRATCH = 128,    // cavity scratch exhausted
+    MERR_INVALID = 256,    // input outside the contract (coordinates, ids)
+};
+
+struct Mesh {
+    const int2* pts;
+    int3* tri;
+    int3* adj;
+    int* sons;
+    int* cflag;
+};
+
+__host__ __device__ inline int& A3(const Mesh& m, int t, int k) { return ((int*)m.adj)[3 * (long long)t + k]; }
+
+// ------------------------------------------------------------------ insertion
+
+struct Ins {
+    Mesh m;
+    int* loc;
+    unsigned long long* claim;
+    int cap;
+    int* err;
+};
+
+// Claim priority: newer epoch first, then on-edge points (they need two
+// triangles), then a hashed rank that spreads choices like a random pick
+// (balanced splits, O(log m) rounds for m points in a triangle), then the id.
+__host__ __device__ inline unsigned long long ins_key(int p, int onedge, unsigned epoch)
+{
+    return ((unsigned long long)epoch << 40) | ((unsigned long long)onedge << 39) |
+           ((unsigned long long)(hash32((uint32_t)p) & 0x7fu) << 32) | (unsigned)p;
+}
+
+__host__ __device__ inline void ins_claim(const Ins& s, int p, unsigned epoch)
+{
+    int l = s.loc[p];
+    if (l < 0) return;
+    int t = l >> 2, k = l & 3;
+    unsigned long long key = ins_key(p, k < 3, epoch);
+    at_max(&s.claim[t], key);
+    if (k < 3) at_max(&s.claim[A3(s.m, t, k)], key);
+}
+
+// slots needed by p if it won every triangle it claimed (3 interior, 4 edge)
+__host__ __device__ inline int ins_need(const Ins& s, int p, unsigned epoch)
+{
+    int l = s.loc[p];
+    if (l < 0) return 0;
+    int t = l >> 2, k = l & 3;
+    unsigned long long key = ins_key(p, k < 3, epoch);
+    if (s.claim[t] != key) return 0;
+    if (k == 3) return 3;
+    return s.claim[A3(s.m, t, k)] == key ? 4 : 0;
+}
+
+// Split the triangle(s) of winner p into new slots [base, base+need).
+// Reference: addNewTriangle_step2, rawMesh.cu:200-280 (same child layout).
+__host__ __device__ inline void ins_split(const Ins& s, int p, int base)
+{
+    const Mesh& m = s.m;
+    int l = s.loc[p];
+    int t = l >> 2, k = l & 3;
+    int3 tv = m.tri[t];
+    int3 ta = m.adj[t];
+    if (k == 3) {
+        int a = tv.x, b = tv.y, c = tv.z;
+        m.tri[base] = make_int3(p, b, c);
+        m.adj[base] = make_int3(adj_mark(ta.x), base + 1, base + 2);
+        m.tri[base + 1] = make_int3(p, c, a);
+        m.adj[base + 1] = make_int3(adj_mark(ta.y), base + 2, base);
+        m.tri[base + 2] = make_int3(p, a, b);
+        m.adj[base + 2] = make_int3(adj_mark(ta.z), base, base + 1);
+        m.sons[t] = base;
+    } else {
+        int n = get3(ta, k);
+        int o = get3(tv, k), a = get3(tv, nx3(k)), b = get3(tv, pv3(k));
+        int at_a = get3(ta, nx3(k));  // across (b,o)
+        int at_b = get3(ta, pv3(k));  // across (o,a)
+        int3 nv = m.tri[n];
+        int3 na = m.adj[n];
+        int j = slot_other(nv, a, b);
+        int o2 = get3(nv, j);         // n = (o2, b, a) rotated
+        int an_b = get3(na, nx3(j));  // across (a,o2)
+        int an_a = get3(na, pv3(j));  // across (o2,b)
+        // t side: (o,a,p) and (o,p,b)
+        m.tri[base] = make_int3(o, a, p);
+        m.adj[base] = make_int3(base + 3, base + 1, adj_mark(at_b));
+        m.tri[base + 1] = make_int3(o, p, b);
+        m.adj[base + 1] = make_int3(base + 2, adj_mark(at_a), base);
+        // n side: (o2,b,p) and (o2,p,a)
+        m.tri[base + 2] = make_int3(o2, b, p);
+        m.adj[base + 2] = make_int3(base + 1, base + 3, adj_mark(an_a));
+        m.tri[base + 3] = make_int3(o2, p, a);
+        m.adj[base + 3] = make_int3(base, adj_mark(an_b), base + 2);
+        m.sons[t] = -base;
+        m.sons[n] = -(base + 2);
+    }
+    s.loc[p] = -1;
+}
+
+// Resolve the outer adjacency of new slot c after all splits of the round:
+// a marked entry names the parent's old neighbour m; if m was split in the
+// same round, the right child of m is the one holding both edge vertices,
+// otherwise m keeps its slot and gets a back-pointer to c.
+// (The reference instead repairs leaves once at the end by descending the
+// DAG, updateAdjTris_new, rawMesh.cu:590-676.)
+__host__ __device__ inline void ins_fix(const Ins& s, int c)
+{
+    const Mesh& m = s.m;
+    int3 tv = m.tri[c];
+    int3 ta = m.adj[c];
+    for (int k = 0; k < 3; k++) {
+        int a = get3(ta, k);
+        if (!adj_is_mark(a)) continue;
+        int nb = adj_unmark(a);
+        int u = get3(tv, nx3(k)), v = get3(tv, pv3(k));
+        int sn = m.sons[nb];
+        if (sn == -1) {
+            set3(ta, k, nb);
+            A3(m, nb, slot_other(m.tri[nb], u, v)) = c;
+        } else {
+            int c0 = sn > 0 ? sn : -sn, nc = sn > 0 ? 3 : 2;
+            int q = -1;
+            for (int x = c0; x < c0 + nc; x++)
+                if (has2(m.tri[x], u, v)) { q = x; break; }
+            if (q < 0) at_or(s.err, MERR_ADJ);
+            set3(ta, k, q);
+        }
+    }
+    m.adj[c] = ta;
+}
+
+// Closed point-in-triangle: returns -1 outside, 3 strictly inside, or the slot
+// k of the edge (opposite vertex k) that p lies on.
+__host__ __device__ inline int classify(int2 A, int2 B, int2 C, int2 P)
+{
+    int o0 = orient2d(B, C, P);
+    if (o0 < 0) return -1;
+    int o1 = orient2d(C, A, P);
+    if (o1 < 0) return -1;
+    int o2 = orient2d(A, B, P);
+    if (o2 < 0) return -1;
+    int z = (o0 == 0) + (o1 == 0) + (o2 == 0);
+    if (z == 0) return 3;
+    if (z > 1) return -2;  // p is a vertex
+    return o0 == 0 ? 0 : (o1 == 0 ? 1 : 2);
+}
+
+// After the splits: move a still-pending point into the child of its split
+// triangle that contains it.  Returns 1 if p stays pending.
+// Reference: scatterPoints/scatterPoint, rawMesh.cu:338-463.
+__host__ __device__ inline int ins_relocate(const Ins& s, int p)
+{
+    const Mesh& m = s.m;
+    int l = s.loc[p];
+    if (l < 0) return 0;
+    int t = l >> 2;
+    int sn = m.sons[t];
+    if (sn == -1) return 1;  // untouched (an on-edge point keeps its edge: see ins_fix)
+    int c0 = sn > 0 ? sn : -sn, nc = sn > 0 ? 3 : 2;
+    int2 P = m.pts[p];
+    for (int c = c0; c < c0 + nc; c++) {
+        int3 tv = m.tri[c];
+        int r = classify(m.pts[tv.x], m.pts[tv.y], m.pts[tv.z], P);
+        if (r >= 0) {
+            s.loc[p] = c * 4 + r;
+            return 1;
+        }
+        if (r == -2) {
+            at_or(s.err, MERR_DUPLICATE);
+            s.loc[p] = -1;
+            return 0;
+        }
+    }
+    at_or(s.err, MERR_LOCATE);
+    s.loc[p] = -1;
+    return 0;
+}
+
+// ------------------------------------------------------------------ flips
+
+struct Flip {
+    Mesh m;
+    unsigned long long* claim;
+    int* stamp;    // epoch of the last flip of this triangle
+    int* partner;  // the other triangle of that flip
+    int* cand;     // proposed edge slot, -1 none
+    int* err;
+};
+
+__host__ __device__ inline int opposite_vertex(const Mesh& m, int n, int u, int v)
+{
+    int3 nv = m.tri[n];
+    return get3(nv, slot_other(nv, u, v));
+}
+
+// Test the edges of t; propose the first edge that is not locally Delaunay
+// (strict inCircle > 0, never a constrained or boundary edge) by claiming
+// both of its triangles with the edge key.
+// Reference: initNeedFlipFE_new, kernel.cu:2198-2228.
+__host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch)
+{
+    const Mesh& m = f.m;
+    int3 tv = m.tri[t];
+    int3 ta = m.adj[t];
+    int cf = m.cflag ? m.cflag[t] : 0;
+    int2 A = m.pts[tv.x], B = m.pts[tv.y], C = m.pts[tv.z];
+    int cand = -1;
+    for (int k = 0; k < 3; k++) {
+        if ((cf >> k) & 1) continue;
+        int n = get3(ta, k);
+        if (n < 0) continue;
+        int o = opposite_vertex(m, n, get3(tv, nx3(k)), get3(tv, pv3(k)));
+        if (incircle(A, B, C, m.pts[o]) > 0) {
+            cand = k;
+            unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
+            at_max(&f.claim[t], key);
+            at_max(&f.claim[n], key);
+            break;
+        }
+    }
+    f.cand[t] = cand;
+}
+
+// Flip t's proposed edge if it won both triangles.  Returns 0 no proposal,
+// 1 lost (t must be re-tested), 2 flipped (t and *nout changed).
+// t=(p,u,v), n=(o,v,u)  ->  t=(p,u,o), n=(p,o,v); outer edges are marked
+// for flip_fix.  Reference: flipEdgeFE / multiFlipFE, kernel.cu:2681-2710,3548.
+__host__ __device__ inline int flip_apply(const Flip& f, int t, unsigned epoch, int* nout)
+{
+    const Mesh& m = f.m;
+    int k = f.cand[t];
+    if (k < 0) return 0;
+    int3 tv = m.tri[t];
+    int3 ta = m.adj[t];
+    int n = get3(ta, k);
+    unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
+    if (f.claim[t] != key || f.claim[n] != key) return 1;
+    int p = get3(tv, k), u = get3(tv, nx3(k)), v = get3(tv, pv3(k));
+    int3 nv = m.tri[n];
+    int3 na = m.adj[n];
+    int j = slot_other(nv, u, v);
+    int o = get3(nv, j);
+    int nb_uo = get3(na, nx3(j)), nb_ov = get3(na, pv3(j));
+    int nb_pu = get3(ta, pv3(k)), nb_vp = get3(ta, nx3(k));
+    int cft = m.cflag ? m.cflag[t] : 0, cfn = m.cflag ? m.cflag[n] : 0;
+    int c_uo = (cfn >> nx3(j)) & 1, c_ov = (cfn >> pv3(j)) & 1;
+    int c_pu = (cft >> pv3(k)) & 1, c_vp = (cft >> nx3(k)) & 1;
+    m.tri[t] = make_int3(p, u, o);
+    m.adj[t] = make_int3(adj_mark(nb_uo), n, adj_mark(nb_pu));
+    m.tri[n] = make_int3(p, o, v);
+    m.adj[n] = make_int3(adj_mark(nb_ov), adj_mark(nb_vp), t);
+    if (m.cflag) {
+        m.cflag[t] = c_uo | (c_pu << 2);
+        m.cflag[n] = c_ov | (c_vp << 1);
+    }
+    f.stamp[t] = (int)epoch;
+    f.stamp[n] = (int)epoch;
+    f.partner[t] = n;
+    f.partner[n] = t;
+    *nout = n;
+    return 2;
+}
+
+// Resolve the marked outer adjacency of a triangle flipped this epoch.
+__host__ __device__ inline void flip_fix(const Flip& f, int t, unsigned epoch)
+{
+    const Mesh& m = f.m;
+    int3 tv = m.tri[t];
+    int3 ta = m.adj[t];
+    for (int k = 0; k < 3; k++) {
+        int a = get3(ta, k);
+        if (!adj_is_mark(a)) continue;
+        int nb = adj_unmark(a);
+        int u = get3(tv, nx3(k)), v = get3(tv, pv3(k));
+        if (f.stamp[nb] == (int)epoch) {
+            int q = has2(m.tri[nb], u, v) ? nb : f.partner[nb];
+            set3(ta, k, q);
+        } else {
+            set3(ta, k, nb);
+            A3(m, nb, slot_other(m.tri[nb], u, v)) = t;
+        }
+    }
+    m.adj[t] = ta;
+}
+
+}  // namespace gcdt

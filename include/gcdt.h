@@ -1,0 +1,186 @@
+/*
+ * gcdt.h -- C ABI of the MI355X (gfx950) constrained Delaunay triangulator.
+ *
+ * Plain C: pointers and sizes only.  Every array argument is a DEVICE pointer
+ * unless the name says host.  Points are on the reference's 2^30 integer grid
+ * (reference: include/math_base.cuh:19 logPointRes=30, kernel.cu:171-188) and
+ * lie strictly inside (0, 2^30-1)^2; the 4 square corners are vertex ids
+ * numPoints..numPoints+3.  Triangles are counter-clockwise vertex triples;
+ * adjacency component k is the neighbour across the edge opposite vertex k
+ * (-1 none); sons is the insertion DAG (-1 leaf) -- the reference layout of
+ * rawMesh.cu:5-22,200-280.
+ *
+ * Every entry point returns a status (GCDT_OK = 0) instead of the reference's
+ * elapsed-time return value; the elapsed GPU milliseconds go to *ms when ms
+ * is not NULL.  Buffers the library allocates come from the MemCache pool and
+ * are released with gcdt_release(ptr, bytes) (the reference's
+ * utils::release(ptr, count), include/utils.cuh:33-39).
+ *
+ * The (tmp, tmpBytes) pair is the reference's caller-provided CUB scratch
+ * (kernel.cu:5386-5387).  Here it is optional: when tmp is NULL the library
+ * takes its scan scratch from the pool; when it is given and smaller than
+ * gcdt_scratch_bytes(n) the call fails with GCDT_ERR_SCRATCH.
+ */
+#ifndef GCDT_H
+#define GCDT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GCDT_OK 0
+#define GCDT_ERR_OOM 1            /* device allocation failed */
+#define GCDT_ERR_INVALID 2        /* bad argument (range, ids, sizes) */
+#define GCDT_ERR_NOT_CONVERGED 3  /* an iterative phase hit its round cap */
+#define GCDT_ERR_DEVICE 4         /* HIP runtime error */
+#define GCDT_ERR_CROSSING 5       /* two constraint segments cross */
+#define GCDT_ERR_SCRATCH 6        /* caller scratch too small */
+#define GCDT_ERR_LOCATE 7         /* a point could not be located / duplicate point */
+#define GCDT_ERR_CAPACITY 8       /* triangle slot capacity exceeded */
+#define GCDT_ERR_INTERNAL 9       /* mesh consistency check failed */
+
+typedef struct { int32_t x, y; } gcdt_int2;
+typedef struct { int32_t x, y, z; } gcdt_int3;
+
+/* ---------------------------------------------------------------- runtime */
+
+/* Last error message of the calling thread (static storage). */
+const char* gcdt_last_error(void);
+/* Number of visible devices; selects device 0 on first use. */
+int gcdt_device_count(int* count);
+int gcdt_synchronize(void);
+
+/* MemCache pool (reference: MemCache.cpp:7-30,87-99 getMemByAbtrtSize /
+ * releaseAbtrtMem; include/utils.cuh:33-39,65-68 utils::malloc/release). */
+int gcdt_alloc(void** ptr, size_t bytes);
+int gcdt_release(void* ptr, size_t bytes);
+int gcdt_pool_init(void);                 /* MemCache::initAbtrtMem */
+int gcdt_pool_stats(size_t* in_use, size_t* cached, size_t* peak);
+int gcdt_pool_set_budget(size_t bytes);
+int gcdt_memcpy_h2d(void* dst, const void* host_src, size_t bytes);
+int gcdt_memcpy_d2h(void* host_dst, const void* src, size_t bytes);
+int gcdt_memcpy_d2d(void* dst, const void* src, size_t bytes);
+int gcdt_memset(void* dst, int value, size_t bytes);
+
+/* scratch bytes the (tmp, tmpBytes) argument must offer for n elements */
+size_t gcdt_scratch_bytes(int64_t n);
+
+/* Live kernel timing: while enabled, launches of the instrumented kernels
+ * (ins_claim, ins_split, ins_fix, ins_reloc, flip_test, flip_apply,
+ * flip_fix, locate_bitmap) are bracketed by hipEvents on their stream.
+ * gcdt_prof_read returns the summed milliseconds, launch count and items
+ * processed (list entries) of one kernel; gcdt_prof_names writes the
+ * ';'-separated names seen so far and returns the full length. */
+int gcdt_prof_enable(int on);
+int gcdt_prof_reset(void);
+int gcdt_prof_read(const char* kernel, double* total_ms, int64_t* launches, int64_t* items);
+int gcdt_prof_names(char* buf, int cap);
+
+/* ------------------------------------------------- reference entry points */
+
+/* rawMesh.h:6, rawMesh.cu:713-746.  One point per occupied 256x256 cell of
+ * the 2^30 grid (the smallest point id in the cell).  Allocates
+ * *samplePoints (gcdt_int2[65536+4]) and *sample (int[65536+4]) from the pool. */
+int gcdt_get_sample_points(void* tmp, size_t tmpBytes, const gcdt_int2* points, int numPoints,
+                           gcdt_int2** samplePoints, int** sample, int* num, double* ms);
+
+/* rawMesh.h:4, rawMesh.cu:1040-1113.  Writes the 4 corners to
+ * points[numPoints..+3] and inserts all points by parallel 1->3 / 1->2
+ * splits of the 2 corner triangles.  tris/sons capacity 4*numPoints+4. */
+int gcdt_raw_mesh_before(void* tmp, size_t tmpBytes, gcdt_int2* points, gcdt_int3* tris, int* sons,
+                         int* numTriangles, int numPoints, double* ms);
+
+/* rawMesh.h:5, rawMesh.cu:994-1038.  tris[0..*numTriangles) with adjTris is
+ * a triangulation of some of the points (the converted sample mesh); the
+ * remaining points are located with the bitmap and inserted.  Leaves
+ * (sons == -1) end with exact adjacency.  Capacity 4*numPoints+4 slots. */
+int gcdt_raw_mesh_after(void* tmp, size_t tmpBytes, gcdt_int2* points, gcdt_int3* tris,
+                        gcdt_int3* adjTris, int* sons, const int* bitmapOffsets, const int* bitmapTriIds,
+                        int* numTriangles, int numPoints, double* ms);
+
+/* rawMesh.h:7, rawMesh.cu:1114-1189.  rawMeshBefore with adjacency. */
+int gcdt_raw_mesh(void* tmp, size_t tmpBytes, gcdt_int2* points, gcdt_int3* tris, gcdt_int3* adjTris,
+                  int* sons, int* numTriangles, int numPoints, double* ms);
+
+/* boostmap.h:3, boostmap.cu:146-181.  CSR over the 1024x1024 cells of the
+ * 2^30 grid: cell (x>>20)*1024+(y>>20) lists every triangle whose closed
+ * region meets the closed cell square.  Allocates *bitmapOffsets
+ * (int[2^20+1]) and *bitmapTris (int[(*bitmapOffsets)[2^20]]). */
+int gcdt_get_bitmap(void* tmp, size_t tmpBytes, const gcdt_int2* points, const gcdt_int3* tris,
+                    int** bitmapOffsets, int** bitmapTris, int numTriangles, double* ms);
+
+/* kernel.cu:5293-5324.  Compacts leaves (sons == -1) of tris into newTris
+ * (and adjTris into newAdjTris with ids remapped, when both are non-NULL). */
+int gcdt_filter_triangles(void* tmp, size_t tmpBytes, const gcdt_int3* tris, gcdt_int3* newTris,
+                          const gcdt_int3* adjTris, gcdt_int3* newAdjTris, const int* sons,
+                          int* numTriangles, double* ms);
+
+/* kernel.cu:833-860.  Sample-local vertex ids -> global ids; the sample's
+ * corners numSamples..+3 -> numPoints..+3. */
+int gcdt_convert_sample(gcdt_int3* tris, const int* sampleId, int numSamples, int numPoints, int numTriangles);
+
+/* kernel.cu:5038-5107.  Flips a triangulation of points[0..numPoints) to
+ * the Delaunay triangulation in place and writes its adjacency. */
+int gcdt_delaunay_sample(void* tmp, size_t tmpBytes, const gcdt_int2* points, gcdt_int3* tris,
+                         gcdt_int3* adjTris, int numPoints, int numTriangles, double* ms);
+
+/* fastDelaunay.h:5, fastDelaunay.cu:710-863 (a flip-only host prototype in
+ * the reference that never writes back).  Here: flips the given mesh to
+ * Delaunay on the GPU, in place, updating adjTris. */
+int gcdt_try_fast_delaunay(void* tmp, size_t tmpBytes, const gcdt_int2* points, gcdt_int3* tris,
+                           gcdt_int3* adjTris, int numPoints, int numTriangles, double* ms);
+
+/* kernel.cu:4697-4860.  Constrained Delaunay flipping of tris (no adjacency
+ * given): builds adjacency, marks every triangle edge that equals a
+ * constraint as unflippable, and flips all other edges to local Delaunay.
+ * Unlike the reference it does not Morton-reorder points, so *points and
+ * *tris keep their addresses and vertex ids. */
+int gcdt_delaunay_new(void* tmp, size_t tmpBytes, gcdt_int2** points, gcdt_int3** tris,
+                      const gcdt_int2* cons, int numPoints, int numTriangles, int numCons, double* ms);
+
+/* addConstraints.h:5, addConstraints.cu:1723-1835.  Recovers every
+ * constraint (vertex-id pair) in the leaf mesh of tris/adjTris/sons.  A
+ * segment that passes exactly through vertices is split at them (the
+ * reference drops it, addConstraints.cu:962-965).  Triangles crossed by a
+ * segment are retired (sons = 0) and their replacement triangles are returned
+ * in *newTris (allocated from the pool, *numNewTris entries), exactly as the
+ * reference's caller expects (kernel.cu:5427-5442). */
+int gcdt_add_constraints(void* tmp, size_t tmpBytes, const gcdt_int2* points, gcdt_int3* tris,
+                         gcdt_int3* adjTris, int* sons, const int* bitmapOffsets, const int* bitmapTris,
+                         const gcdt_int2* cons, gcdt_int3** newTris, int numCons, int numPoints,
+                         int numTriangles, int* numNewTris, double* ms);
+
+/* -------------------------------------------------------------- one call */
+
+typedef struct {
+    double ms_total;       /* GPU time of the whole solve */
+    double ms_sample;      /* sampling + sample mesh + sample flips */
+    double ms_bitmap;
+    double ms_insert;      /* location + insertion rounds */
+    double ms_flip;        /* Delaunay flips of the full mesh */
+    double ms_constraints; /* constraint recovery + constrained flips */
+    int insert_rounds;
+    int flip_rounds;
+    int constraint_rounds;
+    int num_triangles;
+    int64_t flips;
+    int64_t segments;      /* constraint pieces after splitting at vertices */
+    int64_t splits;        /* pieces - constraints */
+} gcdt_stats;
+
+/* Full CDT of points[0..N) (device, 2^30 grid, strictly inside the square,
+ * pairwise distinct) with constraints cons[0..C) (device vertex-id pairs,
+ * non-crossing).  Writes 2N+2 triangles to tris_out (device, capacity 2N+2),
+ * vertex ids as given, corners N..N+3.  adj_out (optional, may be NULL)
+ * receives the adjacency.  stream may be NULL (default stream). */
+int gcdt_solve(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcdt_int3* tris_out,
+               gcdt_int3* adj_out, int* num_tris, gcdt_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GCDT_H */

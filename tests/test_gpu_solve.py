@@ -1,0 +1,87 @@
+"""GPU parity of the full CDT solve (gcdt_solve through the C ABI) against
+the oracle: canonical triangle sets must be bit-identical when the CDT is
+unique (no cocircular unconstrained edge), and every output must pass the
+exact validator (tiling, constraint chains, local Delaunay)."""
+import numpy as np
+import pytest
+
+from gcdt_amd import gen
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(g, c):
+    from gcdt_amd import solve
+    return solve(g, c)
+
+
+def _check(g, c, expect_unique=None):
+    tris, st = _solve(g, c)
+    assert st["num_triangles"] == 2 * len(g) + 2
+    v = O.validate(g, tris, c)
+    assert v["valid"], v
+    ref, _ = O.cdt(g, c)
+    vr = O.validate(g, ref, c)
+    if vr["unique"]:
+        assert np.array_equal(O.canonical(tris), O.canonical(ref))
+    if expect_unique is not None:
+        assert vr["unique"] == expect_unique
+    return st
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_small(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 2000))
+    p, _ = gen.dedup(rng.integers(0, 1 << 20, size=(n, 2)))
+    _check(gen.to_grid(p), np.zeros((0, 2), np.int32))
+
+
+def test_tiny_inputs():
+    for n in (1, 2, 3, 4, 5):
+        p = gen.dedup(np.array([[10 + 7 * i, 20 + 3 * i * i] for i in range(n)]))[0]
+        _check(gen.to_grid(p), np.zeros((0, 2), np.int32))
+
+
+def test_uniform_small_constraints():
+    p, c = gen.uniform_cdt(n=20000, c=200)
+    st = _check(gen.to_grid(p), c, expect_unique=True)
+    assert st["segments"] > len(c)  # some segments pass through points and are split
+
+
+def test_config_small():
+    """100k uniform points + 1000 horizontal segments (the CPU-runnable config)."""
+    p, c = gen.uniform_cdt()
+    st = _check(gen.to_grid(p), c, expect_unique=True)
+    assert st["splits"] > 0
+
+
+def test_lattice_degenerate():
+    p, c = gen.lattice_cdt(k=128, c=150, max_len=60)
+    _check(gen.to_grid(p), c, expect_unique=False)
+
+
+def test_clustered_circles():
+    p, c = gen.clustered_cdt(n=200_000, circles=100)
+    _check(gen.to_grid(p), c)
+
+
+def test_star_instance():
+    s = gen.Stream(11)
+    p, c = gen.star_instance(s, 5000)
+    _check(gen.to_grid(p), c)
+
+
+def test_rejects_duplicate_point():
+    from gcdt_amd import GcdtError
+    g = gen.to_grid(np.array([[5, 5], [9, 9], [5, 5], [100, 7]]))
+    with pytest.raises(GcdtError):
+        _solve(g, np.zeros((0, 2), np.int32))
+
+
+def test_rejects_out_of_range():
+    from gcdt_amd import GcdtError
+    g = np.array([[0, 5], [9, 9]], dtype=np.int32)
+    with pytest.raises(GcdtError):
+        _solve(g, np.zeros((0, 2), np.int32))
