@@ -108,12 +108,17 @@ def main(argv=None):
     mpts = n / (ms_per_step / 1000.0) / 1e6
 
     # dominant instrumented kernel and its roofline
+    import ctypes
+    buf = ctypes.create_string_buffer(4096)
+    L.gcdt_prof_names(buf, 4096)
     kern = {}
-    for name in BYTES_PER_ITEM:
+    for name in [x for x in buf.value.decode().split(";") if x]:
         r = _prof(L, name)
         if r:
             kern[name] = r
-    dom = max(kern, key=lambda k: kern[k][0]) if kern else None
+    # dominant kernel among those with a per-item byte model
+    modeled = [k for k in kern if k in BYTES_PER_ITEM]
+    dom = max(modeled, key=lambda k: kern[k][0]) if modeled else None
     roof = None
     if dom:
         ms, launches, items = kern[dom]

@@ -75,6 +75,36 @@ __device__ inline int wave_alloc(int k, int* counter)
     return base + ex;
 }
 
+// Block-wide reservation of k entries per thread with one atomicAdd per block.
+// Every thread of the block must call it (block-uniform control flow).
+template <int BS>
+__device__ inline int block_alloc(int k, int* counter)
+{
+    __shared__ int s_w[BS / 64];
+    __shared__ int s_base;
+    int lane = __lane_id(), w = threadIdx.x >> 6;
+    int wt;
+    int ex = wave_excl_scan(k, wt);
+    if (lane == 0) s_w[w] = wt;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < BS / 64; i++) {
+        int v = s_w[i];
+        if (i < w) pre += v;
+        tot += v;
+    }
+    if (threadIdx.x == 0) s_base = tot > 0 ? atomicAdd(counter, tot) : 0;
+    __syncthreads();
+    int base = s_base;
+    __syncthreads();
+    return base + pre + ex;
+}
+
+// Block-uniform tile loop: each iteration covers `per` consecutive items.
+#define TILE_LOOP(base, n, per) \
+    for (long long base = (long long)blockIdx.x * (per); base < (long long)(n); base += (long long)gridDim.x * (per))
+
 __device__ __host__ inline int nx3(int k) { return k == 2 ? 0 : k + 1; }
 __device__ __host__ inline int pv3(int k) { return k == 0 ? 2 : k - 1; }
 

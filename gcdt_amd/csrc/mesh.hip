@@ -3,6 +3,7 @@
 // leaf compaction and adjacency construction.
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -16,6 +17,7 @@ namespace gcdt {
 
 Ctx::Ctx(hipStream_t s) : st(s)
 {
+    if (const char* e = getenv("GCDT_TAIL_THRESHOLD")) tail_threshold = atoi(e);
     dcnt.alloc(C_NUM);
     GCDT_HIP(hipHostMalloc((void**)&hcnt, sizeof(int) * C_NUM, hipHostMallocDefault));
     zero_counters();
@@ -543,18 +545,31 @@ __global__ __launch_bounds__(kBlock) void k_ins_claim(Ins s, const int* list, co
         ins_claim(s, list[i], epoch);
 }
 
+constexpr int INS_ITEMS = 4;
+
 __global__ __launch_bounds__(kBlock) void k_ins_split(Ins s, const int* list, const int* cnt, unsigned epoch,
                                                       int* ntri)
 {
     int n = *cnt;
-    WAVE_LOOP(i, n)
+    TILE_LOOP(base, n, kBlock * INS_ITEMS)
     {
-        int p = i < n ? list[i] : -1;
-        int need = p >= 0 ? ins_need(s, p, epoch) : 0;
-        int base = wave_alloc(need, ntri);
-        if (need) {
-            if (base + need > s.cap) atomicOr(s.err, MERR_CAPACITY);
-            else ins_split(s, p, base);
+        int pp[INS_ITEMS], need[INS_ITEMS];
+        int tot = 0;
+#pragma unroll
+        for (int j = 0; j < INS_ITEMS; j++) {
+            long long i = base + (long long)j * kBlock + threadIdx.x;
+            pp[j] = i < n ? list[i] : -1;
+            need[j] = pp[j] >= 0 ? ins_need(s, pp[j], epoch) : 0;
+            tot += need[j];
+        }
+        int b = block_alloc<kBlock>(tot, ntri);
+#pragma unroll
+        for (int j = 0; j < INS_ITEMS; j++) {
+            if (need[j]) {
+                if (b + need[j] > s.cap) atomicOr(s.err, MERR_CAPACITY);
+                else ins_split(s, pp[j], b);
+                b += need[j];
+            }
         }
     }
 }
@@ -569,12 +584,84 @@ __global__ __launch_bounds__(kBlock) void k_ins_fix(Ins s, const int* cnt)
 __global__ __launch_bounds__(kBlock) void k_ins_reloc(Ins s, const int* list, const int* cnt, int* list2, int* cnt2)
 {
     int n = *cnt;
-    WAVE_LOOP(i, n)
+    TILE_LOOP(base, n, kBlock * INS_ITEMS)
     {
-        int p = i < n ? list[i] : -1;
-        int keep = p >= 0 ? ins_relocate(s, p) : 0;
-        int slot = wave_alloc(keep, cnt2);
-        if (keep) list2[slot] = p;
+        int keep[INS_ITEMS];
+        int tot = 0;
+#pragma unroll
+        for (int j = 0; j < INS_ITEMS; j++) {
+            long long i = base + (long long)j * kBlock + threadIdx.x;
+            keep[j] = -1;
+            if (i < n) {
+                int p = list[i];
+                if (ins_relocate(s, p)) keep[j] = p;
+            }
+            tot += keep[j] >= 0;
+        }
+        int b = block_alloc<kBlock>(tot, cnt2);
+#pragma unroll
+        for (int j = 0; j < INS_ITEMS; j++)
+            if (keep[j] >= 0) list2[b++] = keep[j];
+    }
+}
+
+// ---- persistent single-workgroup tail: all remaining insertion rounds in one
+// launch (block barriers between the phases, LDS counters for the lists)
+constexpr int TAIL_BS = 1024;
+
+__global__ __launch_bounds__(TAIL_BS) void k_ins_tail(Ins s, int* P, int* P2, int* cnt, int cp, unsigned epoch0,
+                                                      int max_rounds)
+{
+    __shared__ int s_np, s_nt, s_tb;
+    int n = cnt[cp];
+    unsigned ep = epoch0;
+    int r = 0, err = 0;
+    if (threadIdx.x == 0) s_nt = __hip_atomic_load(&cnt[C_NTRI], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    while (n > 0 && r < max_rounds) {
+        for (int i = threadIdx.x; i < n; i += TAIL_BS) ins_claim(s, P[i], ep);
+        if (threadIdx.x == 0) {
+            s_np = 0;
+            s_tb = s_nt;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += TAIL_BS) {
+            int p = P[i];
+            int need = ins_need(s, p, ep);
+            if (need) {
+                int b = atomicAdd(&s_nt, need);
+                if (b + need > s.cap) atomicOr(s.err, MERR_CAPACITY);
+                else ins_split(s, p, b);
+            }
+        }
+        __syncthreads();
+        int tb = s_tb, te = min(s_nt, s.cap);
+        for (int c = tb + (int)threadIdx.x; c < te; c += TAIL_BS) ins_fix(s, c);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += TAIL_BS) {
+            int p = P[i];
+            if (ins_relocate(s, p)) P2[atomicAdd(&s_np, 1)] = p;
+        }
+        __syncthreads();
+        int n2 = s_np;
+        if (n2 >= n && te == tb) {
+            err = 1;
+            break;
+        }
+        n = n2;
+        int* tmp = P;
+        P = P2;
+        P2 = tmp;
+        ep++;
+        r++;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cnt[C_NTRI] = s_nt;
+        cnt[C_TAIL_N] = n;
+        cnt[C_TAIL_R] = r;
+        cnt[C_TAIL_ERR] = err;
+        cnt[C_TAIL_PAR] = r & 1;
     }
 }
 
@@ -587,11 +674,28 @@ int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int n
     int cp = C_NPEND, cp2 = C_NPEND2;
     int r = 0;
     while (npend > 0) {
+        if (npend <= c.tail_threshold) {
+            unsigned ep0 = c.epoch + 1;
+            {
+                ProfScope ps("ins_tail", c.st, npend);
+                k_ins_tail<<<1, TAIL_BS, 0, c.st>>>(s, P, P2, c.dcnt.p, cp, ep0, 1 << 20);
+                GCDT_LAUNCH_CHECK();
+            }
+            c.read(0, C_NUM);
+            if (c.hcnt[C_ERR]) c.check_err();
+            if (c.hcnt[C_TAIL_ERR]) throw Error(GCDT_ERR_INTERNAL, "insertion round without progress (tail)");
+            r += c.hcnt[C_TAIL_R];
+            c.epoch += (unsigned)c.hcnt[C_TAIL_R];
+            if (c.hcnt[C_TAIL_N] != 0) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion tail did not converge");
+            npend = 0;
+            break;
+        }
         if (++r > 100000) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion rounds did not converge");
         unsigned ep = c.next_epoch();
         k_round_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, cp2);
         GCDT_LAUNCH_CHECK();
         int g = grid_for(npend);
+        int g4 = grid_for((npend + INS_ITEMS - 1) / INS_ITEMS);
         {
             ProfScope ps("ins_claim", c.st, npend);
             k_ins_claim<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, ep);
@@ -599,7 +703,7 @@ int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int n
         GCDT_LAUNCH_CHECK();
         {
             ProfScope ps("ins_split", c.st, npend);
-            k_ins_split<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, ep, c.dcnt.p + C_NTRI);
+            k_ins_split<<<g4, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, ep, c.dcnt.p + C_NTRI);
         }
         GCDT_LAUNCH_CHECK();
         {
@@ -609,7 +713,7 @@ int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int n
         GCDT_LAUNCH_CHECK();
         {
             ProfScope ps("ins_reloc", c.st, npend);
-            k_ins_reloc<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, P2, c.dcnt.p + cp2);
+            k_ins_reloc<<<g4, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, P2, c.dcnt.p + cp2);
         }
         GCDT_LAUNCH_CHECK();
         std::swap(P, P2);
@@ -639,31 +743,48 @@ __global__ __launch_bounds__(kBlock) void k_flip_test(Flip f, const int* list, c
 {
     int n = *cnt;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        flip_test(f, list ? list[i] : (int)i, epoch);
+        flip_test(f, list ? list[i] : (int)i, epoch, list == nullptr);
 }
 
+constexpr int FA_ITEMS = 4;  // list items per thread per block-wide append
+
 __global__ __launch_bounds__(kBlock) void k_flip_apply(Flip f, const int* list, const int* cnt, unsigned epoch,
-                                                       int* qround, int* list2, int* cnt2, int* flipped, int* nflip)
+                                                       int* list2, int* cnt2, int* flipped, int* nflip)
 {
     int n = *cnt;
-    WAVE_LOOP(i, n)
+    TILE_LOOP(base, n, kBlock * FA_ITEMS)
     {
-        int t = -1, nn = -1, res = 0;
-        if (i < n) {
-            t = list ? list[i] : (int)i;
-            res = flip_apply(f, t, epoch, &nn);
+        int qa[FA_ITEMS], qb[FA_ITEMS];
+        int nq = 0, nf = 0;
+#pragma unroll
+        for (int j = 0; j < FA_ITEMS; j++) {
+            qa[j] = -1;
+            qb[j] = -1;
+            long long i = base + (long long)j * kBlock + threadIdx.x;
+            if (i < n) {
+                int t = list ? list[i] : (int)i;
+                int nn = -1;
+                int res = flip_apply(f, t, epoch, &nn);
+                if (res == 1 || res == 2) qa[j] = t;
+                if (res == 2) qb[j] = nn;
+            }
+            nq += (qa[j] >= 0) + (qb[j] >= 0);
+            nf += (qb[j] >= 0) ? 2 : 0;
         }
-        int q0 = -1, q1 = -1;
-        if (res >= 1 && atomicMax(&qround[t], (int)epoch) < (int)epoch) q0 = t;
-        if (res == 2 && atomicMax(&qround[nn], (int)epoch) < (int)epoch) q1 = nn;
-        int k = (q0 >= 0) + (q1 >= 0);
-        int b = wave_alloc(k, cnt2);
-        if (q0 >= 0) list2[b++] = q0;
-        if (q1 >= 0) list2[b] = q1;
-        int fb = wave_alloc(res == 2 ? 2 : 0, nflip);
-        if (res == 2) {
-            flipped[fb] = t;
-            flipped[fb + 1] = nn;
+        int b = block_alloc<kBlock>(nq, cnt2);
+        int fb = block_alloc<kBlock>(nf, nflip);
+#pragma unroll
+        for (int j = 0; j < FA_ITEMS; j++) {
+            if (qa[j] >= 0) {
+                list2[b++] = qa[j];
+                f.mark[qa[j]] = (int)epoch + 1;
+            }
+            if (qb[j] >= 0) {
+                list2[b++] = qb[j];
+                f.mark[qb[j]] = (int)epoch + 1;
+                flipped[fb++] = qa[j];
+                flipped[fb++] = qb[j];
+            }
         }
     }
 }
@@ -675,19 +796,88 @@ __global__ __launch_bounds__(kBlock) void k_flip_fix(Flip f, const int* list, co
         flip_fix(f, list[i], epoch);
 }
 
+// Persistent single-workgroup tail of the flip rounds.
+__global__ __launch_bounds__(TAIL_BS) void k_flip_tail(Flip f, int* A, int* B, int* lf, int* cnt, unsigned epoch0,
+                                                       int max_rounds)
+{
+    __shared__ int s_n2, s_nf;
+    int n = cnt[C_NACT];
+    unsigned ep = epoch0;
+    int r = 0, err = 0, flips = 0;
+    while (n > 0 && r < max_rounds) {
+        for (int i = threadIdx.x; i < n; i += TAIL_BS) flip_test(f, A[i], ep, false);
+        if (threadIdx.x == 0) {
+            s_n2 = 0;
+            s_nf = 0;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += TAIL_BS) {
+            int t = A[i], nn = -1;
+            int res = flip_apply(f, t, ep, &nn);
+            if (res == 1) {
+                B[atomicAdd(&s_n2, 1)] = t;
+                f.mark[t] = (int)ep + 1;
+            } else if (res == 2) {
+                int b = atomicAdd(&s_n2, 2);
+                B[b] = t;
+                B[b + 1] = nn;
+                f.mark[t] = (int)ep + 1;
+                f.mark[nn] = (int)ep + 1;
+                int q = atomicAdd(&s_nf, 2);
+                lf[q] = t;
+                lf[q + 1] = nn;
+            }
+        }
+        __syncthreads();
+        int nf = s_nf, n2 = s_n2;
+        for (int i = threadIdx.x; i < nf; i += TAIL_BS) flip_fix(f, lf[i], ep);
+        __syncthreads();
+        if (n2 > 0 && nf == 0) {
+            err = 1;
+            break;
+        }
+        flips += nf / 2;
+        n = n2;
+        int* tmp = A;
+        A = B;
+        B = tmp;
+        ep++;
+        r++;
+    }
+    if (threadIdx.x == 0) {
+        cnt[C_TAIL_N] = n;
+        cnt[C_TAIL_R] = r;
+        cnt[C_TAIL_ERR] = err;
+        cnt[C_TAIL_F] = flips;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mark_list(const int* list, int n, int* mark, int v)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        mark[list[i]] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_iota(int* p, int n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = (int)i;
+}
+
 long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds)
 {
     DBuf<unsigned long long> claim(T);
-    DBuf<int> stamp(T), partner(T), cand(T), qround(T), la(T), lb(T), lf(T);
+    DBuf<int> stamp(T), partner(T), cand(T), mark(T), la(T), lb(T), lf(T);
     fill_u64(c, claim.p, 0, T);
     fill_i32(c, stamp.p, 0, T);
-    fill_i32(c, qround.p, 0, T);
+    fill_i32(c, mark.p, 0, T);
     Flip f;
     f.m = m;
     f.claim = claim.p;
     f.stamp = stamp.p;
     f.partner = partner.p;
     f.cand = cand.p;
+    f.mark = mark.p;
     f.err = c.dcnt.p + C_ERR;
     int nact = init ? ninit : T;
     const int* act = init;
@@ -696,6 +886,8 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
     // implicit list: the count lives in C_CONST; explicit: in C_NACT
     if (init) {
         c.set(C_NACT, ninit);
+        k_mark_list<<<grid_for(ninit), kBlock, 0, c.st>>>(init, ninit, mark.p, (int)c.epoch + 1);
+        GCDT_LAUNCH_CHECK();
     } else {
         c.set(C_CONST, T);
     }
@@ -703,6 +895,30 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
     long long flips = 0;
     int r = 0;
     while (nact > 0) {
+        if (nact <= c.tail_threshold) {
+            unsigned ep0 = c.epoch + 1;
+            if (!act) {  // small mesh: materialise the implicit list
+                k_iota<<<grid_for(nact), kBlock, 0, c.st>>>(lb.p, nact);
+                GCDT_LAUNCH_CHECK();
+                fill_i32(c, mark.p, (int)ep0, nact);
+                act = lb.p;
+                nxt = la.p;
+                c.set(C_NACT, nact);
+            }
+            {
+                ProfScope ps("flip_tail", c.st, nact);
+                k_flip_tail<<<1, TAIL_BS, 0, c.st>>>(f, (int*)act, nxt, lf.p, c.dcnt.p, ep0, 1 << 20);
+                GCDT_LAUNCH_CHECK();
+            }
+            c.read(0, C_NUM);
+            if (c.hcnt[C_ERR]) c.check_err();
+            if (c.hcnt[C_TAIL_ERR]) throw Error(GCDT_ERR_INTERNAL, "flip round without progress (tail)");
+            r += c.hcnt[C_TAIL_R];
+            c.epoch += (unsigned)c.hcnt[C_TAIL_R];
+            flips += c.hcnt[C_TAIL_F];
+            if (c.hcnt[C_TAIL_N] != 0) throw Error(GCDT_ERR_NOT_CONVERGED, "flip tail did not converge");
+            break;
+        }
         if (++r > 1000000) throw Error(GCDT_ERR_NOT_CONVERGED, "flip rounds did not converge");
         unsigned ep = c.next_epoch();
         k_flip_begin<<<1, 1, 0, c.st>>>(c.dcnt.p);
@@ -711,19 +927,19 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
         {
             ProfScope ps("flip_test", c.st, nact);
             k_flip_test<<<g, kBlock, 0, c.st>>>(f, act, c.dcnt.p + cnt_idx, ep);
+            GCDT_LAUNCH_CHECK();
         }
-        GCDT_LAUNCH_CHECK();
         {
             ProfScope ps("flip_apply", c.st, nact);
-            k_flip_apply<<<g, kBlock, 0, c.st>>>(f, act, c.dcnt.p + cnt_idx, ep, qround.p, nxt, c.dcnt.p + C_NACT2,
-                                                 lf.p, c.dcnt.p + C_NFLIP);
+            k_flip_apply<<<grid_for((nact + FA_ITEMS - 1) / FA_ITEMS), kBlock, 0, c.st>>>(
+                f, act, c.dcnt.p + cnt_idx, ep, nxt, c.dcnt.p + C_NACT2, lf.p, c.dcnt.p + C_NFLIP);
+            GCDT_LAUNCH_CHECK();
         }
-        GCDT_LAUNCH_CHECK();
         {
             ProfScope ps("flip_fix", c.st, 0);
             k_flip_fix<<<grid_for(nact), kBlock, 0, c.st>>>(f, lf.p, c.dcnt.p + C_NFLIP, ep);
+            GCDT_LAUNCH_CHECK();
         }
-        GCDT_LAUNCH_CHECK();
         c.read(0, 8);
         if (c.hcnt[C_ERR]) c.check_err();
         // the highest-keyed proposal always wins both of its triangles
@@ -734,7 +950,6 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
         act = nxt;
         cnt_idx = C_NACT;
         std::swap(nxt, other);
-        if (c.hcnt[C_ERR]) c.check_err();
     }
     if (rounds) *rounds = r;
     return flips;

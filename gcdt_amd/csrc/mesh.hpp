@@ -73,6 +73,11 @@ enum : int {
     C_SCR = 13,    // cavity scratch cursor
     C_NREW = 14,   // rewritten triangles this round
     C_NWIN = 15,
+    C_TAIL_N = 16,    // persistent tail: remaining list length
+    C_TAIL_R = 17,    // rounds done
+    C_TAIL_ERR = 18,  // 1 = a round without progress
+    C_TAIL_PAR = 19,  // parity: the final list is the second buffer
+    C_TAIL_F = 20,    // flips done
     C_NUM = 32
 };
 
@@ -86,6 +91,8 @@ struct Ctx {
     // instead of the pool when set, and fail with GCDT_ERR_SCRATCH if small
     int* ext_scratch = nullptr;
     size_t ext_scratch_bytes = 0;
+    // lists at or below this length finish in one persistent workgroup
+    int tail_threshold = 4096;
     explicit Ctx(hipStream_t s);
     ~Ctx();
     Ctx(const Ctx&) = delete;

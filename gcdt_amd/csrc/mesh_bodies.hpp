@@ -29,6 +29,13 @@ __device__ inline int at_max(int* p, int v) { return atomicMax(p, v); }
 __device__ inline int at_min(int* p, int v) { return atomicMin(p, v); }
 __device__ inline int at_or(int* p, int v) { return atomicOr(p, v); }
 __device__ inline int at_add(int* p, int v) { return atomicAdd(p, v); }
+// Claim words are only ever changed by device-scope atomics (performed in L2);
+// read them with an agent-scope atomic load so a stale L1 line is never used,
+// also inside a persistent workgroup that loops over rounds.
+__device__ inline unsigned long long ld_u64(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 enum : int {
     MERR_CAPACITY = 1,     // triangle slots exhausted
@@ -63,12 +70,34 @@ struct Ins {
 };
 
 // Claim priority: newer epoch first, then on-edge points (they need two
-// triangles), then a hashed rank that spreads choices like a random pick
-// (balanced splits, O(log m) rounds for m points in a triangle), then the id.
-__host__ __device__ inline unsigned long long ins_key(int p, int onedge, unsigned epoch)
+// triangles), then the point nearest the triangle's centroid (balanced
+// splits: O(log m) rounds for m points in a triangle, and well-shaped children
+// that need few flips later), then the id.  Layout: epoch 20 bits | on-edge 1 |
+// closeness 11 | id 32.
+__host__ __device__ inline unsigned closeness_rank(int2 A, int2 B, int2 C, int2 P, int p)
 {
-    return ((unsigned long long)epoch << 40) | ((unsigned long long)onedge << 39) |
-           ((unsigned long long)(hash32((uint32_t)p) & 0x7fu) << 32) | (unsigned)p;
+    long long dx = 3LL * P.x - ((long long)A.x + B.x + C.x);
+    long long dy = 3LL * P.y - ((long long)A.y + B.y + C.y);
+    long long d = (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy);
+    long long size = (long long)(A.x > B.x ? A.x - B.x : B.x - A.x) + (A.y > B.y ? A.y - B.y : B.y - A.y) +
+                     (long long)(B.x > C.x ? B.x - C.x : C.x - B.x) + (B.y > C.y ? B.y - C.y : C.y - B.y) +
+                     (long long)(C.x > A.x ? C.x - A.x : A.x - C.x) + (C.y > A.y ? C.y - A.y : A.y - C.y);
+    // class 3: well inside the triangle ... 0: near its boundary; random within a class
+    unsigned cls = d * 8 < size ? 3u : (d * 4 < size ? 2u : (d * 2 < size ? 1u : 0u));
+    return (cls << 9) | (hash32((uint32_t)p) & 511u);
+}
+
+__host__ __device__ inline unsigned long long ins_key(int p, int onedge, unsigned close, unsigned epoch)
+{
+    return ((unsigned long long)(epoch & 0xfffffu) << 44) | ((unsigned long long)onedge << 43) |
+           ((unsigned long long)(close & 2047u) << 32) | (unsigned)p;
+}
+
+__host__ __device__ inline unsigned long long ins_key_of(const Ins& s, int p, int t, int k, unsigned epoch)
+{
+    int3 tv = s.m.tri[t];
+    unsigned close = closeness_rank(s.m.pts[tv.x], s.m.pts[tv.y], s.m.pts[tv.z], s.m.pts[p], p);
+    return ins_key(p, k < 3, close, epoch);
 }
 
 __host__ __device__ inline void ins_claim(const Ins& s, int p, unsigned epoch)
@@ -76,7 +105,7 @@ __host__ __device__ inline void ins_claim(const Ins& s, int p, unsigned epoch)
     int l = s.loc[p];
     if (l < 0) return;
     int t = l >> 2, k = l & 3;
-    unsigned long long key = ins_key(p, k < 3, epoch);
+    unsigned long long key = ins_key_of(s, p, t, k, epoch);
     at_max(&s.claim[t], key);
     if (k < 3) at_max(&s.claim[A3(s.m, t, k)], key);
 }
@@ -87,10 +116,10 @@ __host__ __device__ inline int ins_need(const Ins& s, int p, unsigned epoch)
     int l = s.loc[p];
     if (l < 0) return 0;
     int t = l >> 2, k = l & 3;
-    unsigned long long key = ins_key(p, k < 3, epoch);
-    if (s.claim[t] != key) return 0;
+    unsigned long long key = ins_key_of(s, p, t, k, epoch);
+    if (ld_u64(&s.claim[t]) != key) return 0;
     if (k == 3) return 3;
-    return s.claim[A3(s.m, t, k)] == key ? 4 : 0;
+    return ld_u64(&s.claim[A3(s.m, t, k)]) == key ? 4 : 0;
 }
 
 // Split the triangle(s) of winner p into new slots [base, base+need).
@@ -225,6 +254,7 @@ struct Flip {
     int* stamp;    // epoch of the last flip of this triangle
     int* partner;  // the other triangle of that flip
     int* cand;     // proposed edge slot, -1 none
+    int* mark;     // epoch in which the triangle is in the active list
     int* err;
 };
 
@@ -238,7 +268,11 @@ __host__ __device__ inline int opposite_vertex(const Mesh& m, int n, int u, int 
 // (strict inCircle > 0, never a constrained or boundary edge) by claiming
 // both of its triangles with the edge key.
 // Reference: initNeedFlipFE_new, kernel.cu:2198-2228.
-__host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch)
+// An edge whose two triangles are both active is tested by the lower id only
+// (all_active: every triangle is active this epoch).  Every illegal edge keeps
+// an active triangle: the lower one proposes an edge, so it is re-queued or
+// flipped.
+__host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch, bool all_active)
 {
     const Mesh& m = f.m;
     int3 tv = m.tri[t];
@@ -250,6 +284,7 @@ __host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch)
         if ((cf >> k) & 1) continue;
         int n = get3(ta, k);
         if (n < 0) continue;
+        if (n < t && (all_active || f.mark[n] == (int)epoch)) continue;
         int o = opposite_vertex(m, n, get3(tv, nx3(k)), get3(tv, pv3(k)));
         if (incircle(A, B, C, m.pts[o]) > 0) {
             cand = k;
@@ -263,7 +298,8 @@ __host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch)
 }
 
 // Flip t's proposed edge if it won both triangles.  Returns 0 no proposal,
-// 1 lost (t must be re-tested), 2 flipped (t and *nout changed).
+// 1 lost (t must be re-tested), 2 flipped (t and *nout changed), 3 lost to a
+// proposal that wins and flips t (its winner re-queues t, so t does not).
 // t=(p,u,v), n=(o,v,u)  ->  t=(p,u,o), n=(p,o,v); outer edges are marked
 // for flip_fix.  Reference: flipEdgeFE / multiFlipFE, kernel.cu:2681-2710,3548.
 __host__ __device__ inline int flip_apply(const Flip& f, int t, unsigned epoch, int* nout)
@@ -275,7 +311,16 @@ __host__ __device__ inline int flip_apply(const Flip& f, int t, unsigned epoch, 
     int3 ta = m.adj[t];
     int n = get3(ta, k);
     unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
-    if (f.claim[t] != key || f.claim[n] != key) return 1;
+    unsigned long long ct = ld_u64(&f.claim[t]);
+    if (ct != key || ld_u64(&f.claim[n]) != key) {
+        if (ct != key) {
+            // ct was proposed by triangle t2 for the edge it shares with t; t2
+            // wins (and flips t) iff it also holds its own claim
+            int t2 = (int)((ct & 0xffffffffu) / 3);
+            if ((ct >> 32) == epoch && ld_u64(&f.claim[t2]) == ct) return 3;
+        }
+        return 1;
+    }
     int p = get3(tv, k), u = get3(tv, nx3(k)), v = get3(tv, pv3(k));
     int3 nv = m.tri[n];
     int3 na = m.adj[n];
