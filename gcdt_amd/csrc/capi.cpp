@@ -184,18 +184,27 @@ struct SolveOut {
 };
 
 // Full pipeline on device arrays; pts holds N+4 entries (corners written here).
-static void solve_impl(Ctx& c, int2* pts, int N, const int2* cons, int C, int3* tris_out, int3* adj_out,
-                       SolveOut& out)
+static void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, int3* tris_out,
+                       int3* adj_out, SolveOut& out)
 {
     gcdt_stats& st = out.st;
     Timer total(c.st);
-    check_points(c, pts, N);
+    check_points(c, pts_in, N);
+    // work in Morton order for locality; ids are mapped back at the end
+    DBuf<int2> pts_buf((size_t)N + 4);
+    DBuf<int> perm(N > 0 ? N : 1), inv(N > 0 ? N : 1);
+    DBuf<int2> cons_buf(C > 0 ? C : 1);
+    int2* pts = pts_buf.p;
+    morton_order(c, pts_in, N, pts, perm.p, inv.p);
+    relabel_pairs(c, cons_in, C, inv.p, N, cons_buf.p);
+    const int2* cons = cons_buf.p;
     write_corners(c, pts, N);
     // ---- 1. sample + sample mesh
     Timer t1(c.st);
     DBuf<int> sid(1 << 16);
     int S = sample_points(c, pts, N, sid.p, nullptr);
     DBuf<int> loc((size_t)N + 4), pend((size_t)(N > 1 ? N : 1)), pend2((size_t)(N > 1 ? N : 1));
+    DBuf<unsigned long long> pkey((size_t)N + 4);
     long long capA = 4LL * S + 8;
     DBuf<int3> tA(capA), aA(capA);
     DBuf<int> sA(capA);
@@ -208,6 +217,7 @@ static void solve_impl(Ctx& c, int2* pts, int N, const int2* cons, int C, int3* 
     ia.m = Mesh{pts, tA.p, aA.p, sA.p, nullptr};
     ia.loc = loc.p;
     ia.claim = clA.p;
+    ia.pkey = pkey.p;
     ia.cap = (int)capA;
     ia.err = c.dcnt.p + C_ERR;
     int rs = 0;
@@ -243,16 +253,18 @@ static void solve_impl(Ctx& c, int2* pts, int N, const int2* cons, int C, int3* 
     ib.m = Mesh{pts, tB.p, aB.p, sB.p, nullptr};
     ib.loc = loc.p;
     ib.claim = clB.p;
+    ib.pkey = pkey.p;
     ib.cap = (int)capB;
     ib.err = c.dcnt.p + C_ERR;
     int ri = 0;
     int ntB = insert_rounds(c, ib, pend, pend2, np, Ts, &ri);
     st.insert_rounds = rs + ri;
     clB.release();
+    pkey.release();
     loc.release();
     pend.release();
     pend2.release();
-    int T = filter_leaves(c, tB.p, aB.p, sB.p, ntB, tris_out, adj_out, 2LL * N + 2);
+    int T = filter_leaves_morton(c, pts, tB.p, aB.p, sB.p, ntB, tris_out, adj_out, 2LL * N + 2);
     tB.release();
     aB.release();
     sB.release();
@@ -267,17 +279,24 @@ static void solve_impl(Ctx& c, int2* pts, int N, const int2* cons, int C, int3* 
     // ---- 5. constraints
     Timer t5(c.st);
     if (C > 0) {
-        DBuf<int> cflag(T);
+        DBuf<int> cflag(T), touched(T);
         fill_i32(c, cflag.p, 0, T);
+        fill_i32(c, touched.p, 0, T);
         Mesh m{pts, tris_out, adj_out, nullptr, cflag.p};
-        ConsResult cr = insert_constraints(c, m, T, N + 4, cons, C, nullptr, nullptr);
+        ConsResult cr = insert_constraints(c, m, T, N + 4, cons, C, touched.p, nullptr);
         st.constraint_rounds = cr.rounds;
         st.segments = cr.pieces;
         st.splits = cr.pieces - C;
-        int f3 = 0;
-        st.flips += flip_rounds(c, m, T, nullptr, 0, &f3);
-        st.flip_rounds += f3;
+        // only edges of rewritten triangles can be non-Delaunay now
+        DBuf<int> tl(T);
+        int nt = flags_to_list(c, touched.p, T, tl.p);
+        if (nt > 0) {
+            int f3 = 0;
+            st.flips += flip_rounds(c, m, T, tl.p, nt, &f3);
+            st.flip_rounds += f3;
+        }
     }
+    unmap_tris(c, tris_out, T, perm.p, N);
     st.ms_constraints = t5.ms();
     st.ms_total = total.ms();
     st.num_triangles = T;
@@ -295,8 +314,6 @@ extern "C" int gcdt_solve(const gcdt_int2* points, int N, const gcdt_int2* cons,
     if (N > (1 << 28)) return fail(GCDT_ERR_INVALID, "too many points");
     hipStream_t st = (hipStream_t)stream;
     Ctx c(st);
-    DBuf<int2> pts((size_t)N + 4);
-    if (N > 0) GCDT_HIP(hipMemcpyAsync(pts.p, points, sizeof(int2) * (size_t)N, hipMemcpyDeviceToDevice, st));
     DBuf<int3> adj_tmp;
     int3* adj = (int3*)adj_out;
     if (!adj) {
@@ -304,7 +321,7 @@ extern "C" int gcdt_solve(const gcdt_int2* points, int N, const gcdt_int2* cons,
         adj = adj_tmp.p;
     }
     SolveOut out;
-    solve_impl(c, pts.p, N, (const int2*)cons, C, (int3*)tris_out, adj, out);
+    solve_impl(c, (const int2*)points, N, (const int2*)cons, C, (int3*)tris_out, adj, out);
     if (num_tris) *num_tris = out.T;
     if (stats) *stats = out.st;
     return GCDT_OK;
@@ -379,11 +396,13 @@ static void raw_mesh_impl(Ctx& c, int2* pts, int3* tris, int3* adj_out, int* son
     DBuf<unsigned long long> claim(cap);
     fill_u64(c, claim.p, 0, cap);
     DBuf<int> loc((size_t)n + 4), pend(n > 0 ? n : 1), pend2(n > 0 ? n : 1);
+    DBuf<unsigned long long> pkey((size_t)n + 4);
     locate_two(c, pts, n, loc.p, pend.p);
     Ins s;
     s.m = Mesh{pts, tris, adj, sons, nullptr};
     s.loc = loc.p;
     s.claim = claim.p;
+    s.pkey = pkey.p;
     s.cap = (int)cap;
     s.err = c.dcnt.p + C_ERR;
     int r = 0;
@@ -463,12 +482,13 @@ int gcdt_raw_mesh_after(void* tmp, size_t tmpBytes, gcdt_int2* points, gcdt_int3
     fill_i32(c, isv.p, 0, n);
     int np = locate_bitmap(c, (const int2*)points, n, (const int3*)tris, T0, bitmapOffsets, bitmapTriIds, loc.p,
                            isv.p, pend.p);
-    DBuf<unsigned long long> claim(cap);
+    DBuf<unsigned long long> claim(cap), pkey((size_t)n + 4);
     fill_u64(c, claim.p, 0, cap);
     Ins s;
     s.m = Mesh{(const int2*)points, (int3*)tris, (int3*)adjTris, sons, nullptr};
     s.loc = loc.p;
     s.claim = claim.p;
+    s.pkey = pkey.p;
     s.cap = (int)cap;
     s.err = c.dcnt.p + C_ERR;
     int r = 0;

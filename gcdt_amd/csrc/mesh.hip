@@ -18,14 +18,34 @@ namespace gcdt {
 Ctx::Ctx(hipStream_t s) : st(s)
 {
     if (const char* e = getenv("GCDT_TAIL_THRESHOLD")) tail_threshold = atoi(e);
+    if (const char* e = getenv("GCDT_PIVOT")) pivot = atoi(e);
+    if (const char* e = getenv("GCDT_SINGLE_TESTER")) single_tester = atoi(e);
     dcnt.alloc(C_NUM);
     GCDT_HIP(hipHostMalloc((void**)&hcnt, sizeof(int) * C_NUM, hipHostMallocDefault));
+    GCDT_HIP(hipHostMalloc((void**)&hring, sizeof(int) * 16, hipHostMallocDefault));
+    GCDT_HIP(hipEventCreateWithFlags(&ring_ev[0], hipEventDisableTiming));
+    GCDT_HIP(hipEventCreateWithFlags(&ring_ev[1], hipEventDisableTiming));
     zero_counters();
 }
 
 Ctx::~Ctx()
 {
     if (hcnt) (void)hipHostFree(hcnt);
+    if (hring) (void)hipHostFree(hring);
+    for (auto& e : ring_ev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+void Ctx::snapshot(int slot)
+{
+    GCDT_HIP(hipMemcpyAsync(hring + 8 * slot, dcnt.p, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
+    GCDT_HIP(hipEventRecord(ring_ev[slot], st));
+}
+
+const int* Ctx::wait_snapshot(int slot)
+{
+    GCDT_HIP(hipEventSynchronize(ring_ev[slot]));
+    return hring + 8 * slot;
 }
 
 void Ctx::read(int i, int k)
@@ -427,6 +447,90 @@ int sample_points(Ctx& c, const int2* pts, int n, int* sample_ids, int2* sample_
     return c.get(C_TOTAL);
 }
 
+// ================================================================== Morton order
+
+// 20-bit Morton code of the 1024x1024 cell of a 2^30-grid point
+__device__ inline unsigned morton_cell(int2 p)
+{
+    unsigned x = (unsigned)p.x >> 20, y = (unsigned)p.y >> 20;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    y = (y | (y << 8)) & 0x00FF00FFu;
+    y = (y | (y << 4)) & 0x0F0F0F0Fu;
+    y = (y | (y << 2)) & 0x33333333u;
+    y = (y | (y << 1)) & 0x55555555u;
+    return x | (y << 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n, int* cnt)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[morton_cell(pts[i])], 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton_scatter(const int2* pts, int n, int* cursor, int2* out, int* perm,
+                                                           int* inv)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        int2 p = pts[i];
+        int slot = atomicAdd(&cursor[morton_cell(p)], 1);
+        out[slot] = p;
+        perm[slot] = (int)i;
+        inv[i] = slot;
+    }
+}
+
+void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv)
+{
+    if (n <= 0) return;
+    DBuf<int> cnt(BITMAP_CELLS + 1);
+    fill_i32(c, cnt.p, 0, BITMAP_CELLS + 1);
+    k_morton_count<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cnt.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl(c, cnt.p, cnt.p, BITMAP_CELLS);
+    k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cnt.p, out, perm, inv);
+    GCDT_LAUNCH_CHECK();
+    c.get(C_TOTAL);  // keeps cnt alive until the scatter has run
+}
+
+__global__ __launch_bounds__(kBlock) void k_relabel_pairs(const int2* in, int C, const int* inv, int n, int2* out)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < C; i += (long long)gridDim.x * blockDim.x) {
+        int2 e = in[i];
+        // ids outside [0,n) are kept and rejected later by the constraint checks
+        e.x = (e.x >= 0 && e.x < n) ? inv[e.x] : e.x;
+        e.y = (e.y >= 0 && e.y < n) ? inv[e.y] : e.y;
+        out[i] = e;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_unmap_tris(int3* tri, int T, const int* perm, int n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        int3 t = tri[i];
+        t.x = t.x < n ? perm[t.x] : t.x;
+        t.y = t.y < n ? perm[t.y] : t.y;
+        t.z = t.z < n ? perm[t.z] : t.z;
+        tri[i] = t;
+    }
+}
+
+void relabel_pairs(Ctx& c, const int2* in, int C, const int* inv, int n, int2* out)
+{
+    if (C <= 0) return;
+    k_relabel_pairs<<<grid_for(C), kBlock, 0, c.st>>>(in, C, inv, n, out);
+    GCDT_LAUNCH_CHECK();
+}
+
+void unmap_tris(Ctx& c, int3* tri, int T, const int* perm, int n)
+{
+    if (T <= 0 || n <= 0) return;
+    k_unmap_tris<<<grid_for(T), kBlock, 0, c.st>>>(tri, T, perm, n);
+    GCDT_LAUNCH_CHECK();
+}
+
 // ================================================================== location
 
 // initTriangle, rawMesh.cu:37-57: side of the anti-diagonal (R,0)-(0,R)
@@ -532,17 +636,47 @@ int locate_bitmap(Ctx& c, const int2* pts, int n, const int3* tri, int T, const 
 
 // ================================================================== insertion rounds
 
-__global__ void k_round_begin(int* cnt, int clear_idx)
+// Round prologue on the device: the previous round's output list becomes the
+// input (count C_NPEND2 -> C_NPEND) and the slot base is recorded.
+__global__ void k_round_begin(int* cnt, int move)
 {
+    if (move) cnt[C_NPEND] = cnt[C_NPEND2];
+    cnt[C_NPEND2] = 0;
     cnt[C_TBASE] = cnt[C_NTRI];
-    cnt[clear_idx] = 0;
+}
+
+// Wave-combined claims: the lanes that target the same triangle as the wave's
+// first active lane reduce their keys with shuffles and issue one atomic, so a
+// triangle holding thousands of pending points (early sample rounds, dense
+// clusters) does not serialise on one address.
+__device__ inline void claim_combined(unsigned long long* claim, int t, unsigned long long key)
+{
+    unsigned long long act = __ballot(t >= 0);
+    if (act == 0) return;
+    int leader = __ffsll((long long)act) - 1;
+    int tl = __shfl(t, leader, 64);
+    bool mine = t >= 0 && t == tl;
+    unsigned long long v = mine ? key : 0ull;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        unsigned long long o = __shfl_xor(v, d, 64);
+        v = o > v ? o : v;
+    }
+    if (__lane_id() == leader) atomicMax(&claim[tl], v);
+    if (t >= 0 && !mine) atomicMax(&claim[t], key);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ins_claim(Ins s, const int* list, const int* cnt, unsigned epoch)
 {
     int n = *cnt;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        ins_claim(s, list[i], epoch);
+    WAVE_LOOP(i, n)
+    {
+        int t = -1, t2 = -1;
+        unsigned long long key = 0;
+        if (i < n) t = ins_key_round(s, list[i], epoch, &key, &t2);
+        claim_combined(s.claim, t, key);
+        if (t2 >= 0) atomicMax(&s.claim[t2], key);
+    }
 }
 
 constexpr int INS_ITEMS = 4;
@@ -559,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void k_ins_split(Ins s, const int* list, co
         for (int j = 0; j < INS_ITEMS; j++) {
             long long i = base + (long long)j * kBlock + threadIdx.x;
             pp[j] = i < n ? list[i] : -1;
-            need[j] = pp[j] >= 0 ? ins_need(s, pp[j], epoch) : 0;
+            need[j] = pp[j] >= 0 ? ins_need<false>(s, pp[j], epoch) : 0;
             tot += need[j];
         }
         int b = block_alloc<kBlock>(tot, ntri);
@@ -665,65 +799,83 @@ __global__ __launch_bounds__(TAIL_BS) void k_ins_tail(Ins s, int* P, int* P2, in
     }
 }
 
-int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int npend, int ntri, int* rounds)
+int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, int npend, int ntri, int* rounds)
 {
+    Ins s = s_in;
+    s.pivot = c.pivot;
     c.set(C_NPEND, npend);
     c.set(C_NTRI, ntri);
-    int* P = pend.p;
+    int* P = pend.p;   // input list of the next round to launch
     int* P2 = pend2.p;
-    int cp = C_NPEND, cp2 = C_NPEND2;
     int r = 0;
-    while (npend > 0) {
-        if (npend <= c.tail_threshold) {
-            unsigned ep0 = c.epoch + 1;
-            {
-                ProfScope ps("ins_tail", c.st, npend);
-                k_ins_tail<<<1, TAIL_BS, 0, c.st>>>(s, P, P2, c.dcnt.p, cp, ep0, 1 << 20);
-                GCDT_LAUNCH_CHECK();
-            }
-            c.read(0, C_NUM);
-            if (c.hcnt[C_ERR]) c.check_err();
-            if (c.hcnt[C_TAIL_ERR]) throw Error(GCDT_ERR_INTERNAL, "insertion round without progress (tail)");
-            r += c.hcnt[C_TAIL_R];
-            c.epoch += (unsigned)c.hcnt[C_TAIL_R];
-            if (c.hcnt[C_TAIL_N] != 0) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion tail did not converge");
-            npend = 0;
-            break;
-        }
+    int est = npend;  // estimate of the next round's input size (exact for the first)
+    bool done = npend == 0;
+    // Rounds are launched one ahead of the host's knowledge of their input
+    // size: each round's counters are snapshot asynchronously and read after
+    // the next round is queued, so the GPU does not wait for the host.
+    auto check_round = [&](const int* h) {
+        if (h[C_ERR]) c.check_err();
+        if (h[C_NPEND] > 0 && h[C_NPEND2] >= h[C_NPEND] && h[C_NTRI] == h[C_TBASE])
+            throw Error(GCDT_ERR_INTERNAL, "insertion round without progress");
+    };
+    while (!done && est > c.tail_threshold) {
         if (++r > 100000) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion rounds did not converge");
         unsigned ep = c.next_epoch();
-        k_round_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, cp2);
+        k_round_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, r > 1);
         GCDT_LAUNCH_CHECK();
-        int g = grid_for(npend);
-        int g4 = grid_for((npend + INS_ITEMS - 1) / INS_ITEMS);
+        int g = grid_for(est);
+        int g4 = grid_for((est + INS_ITEMS - 1) / INS_ITEMS);
         {
-            ProfScope ps("ins_claim", c.st, npend);
-            k_ins_claim<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, ep);
+            ProfScope ps("ins_claim", c.st, est);
+            k_ins_claim<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, ep);
+            GCDT_LAUNCH_CHECK();
         }
-        GCDT_LAUNCH_CHECK();
         {
-            ProfScope ps("ins_split", c.st, npend);
-            k_ins_split<<<g4, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, ep, c.dcnt.p + C_NTRI);
+            ProfScope ps("ins_split", c.st, est);
+            k_ins_split<<<g4, kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, ep, c.dcnt.p + C_NTRI);
+            GCDT_LAUNCH_CHECK();
         }
-        GCDT_LAUNCH_CHECK();
         {
             ProfScope ps("ins_fix", c.st, 0);
-            k_ins_fix<<<grid_for(4LL * npend), kBlock, 0, c.st>>>(s, c.dcnt.p);
+            k_ins_fix<<<grid_for(4LL * est), kBlock, 0, c.st>>>(s, c.dcnt.p);
+            GCDT_LAUNCH_CHECK();
         }
-        GCDT_LAUNCH_CHECK();
         {
-            ProfScope ps("ins_reloc", c.st, npend);
-            k_ins_reloc<<<g4, kBlock, 0, c.st>>>(s, P, c.dcnt.p + cp, P2, c.dcnt.p + cp2);
+            ProfScope ps("ins_reloc", c.st, est);
+            k_ins_reloc<<<g4, kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, P2, c.dcnt.p + C_NPEND2);
+            GCDT_LAUNCH_CHECK();
         }
-        GCDT_LAUNCH_CHECK();
+        c.snapshot(r & 1);
         std::swap(P, P2);
-        std::swap(cp, cp2);
-        c.read(0, 8);
+        if (r >= 2) {
+            const int* h = c.wait_snapshot((r - 1) & 1);  // round r-1 (round r is queued)
+            check_round(h);
+            if (h[C_NPEND2] == 0) done = true;  // round r had no input
+            est = h[C_NPEND2];
+        }
+    }
+    if (r >= 1) {
+        const int* h = c.wait_snapshot(r & 1);
+        check_round(h);
+        if (h[C_NPEND2] == 0) done = true;
+        est = h[C_NPEND2];
+    }
+    if (!done) {
+        // finish in one persistent workgroup; its input is P with the count moved
+        k_round_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, r > 0);
+        GCDT_LAUNCH_CHECK();
+        unsigned ep0 = c.epoch + 1;
+        {
+            ProfScope ps("ins_tail", c.st, est);
+            k_ins_tail<<<1, TAIL_BS, 0, c.st>>>(s, P, P2, c.dcnt.p, C_NPEND, ep0, 1 << 20);
+            GCDT_LAUNCH_CHECK();
+        }
+        c.read(0, C_NUM);
         if (c.hcnt[C_ERR]) c.check_err();
-        // every round inserts at least the globally highest-priority point
-        if (c.hcnt[cp] >= npend && c.hcnt[C_NTRI] == c.hcnt[C_TBASE])
-            throw Error(GCDT_ERR_INTERNAL, "insertion round without progress");
-        npend = c.hcnt[cp];
+        if (c.hcnt[C_TAIL_ERR]) throw Error(GCDT_ERR_INTERNAL, "insertion round without progress (tail)");
+        r += c.hcnt[C_TAIL_R];
+        c.epoch += (unsigned)c.hcnt[C_TAIL_R];
+        if (c.hcnt[C_TAIL_N] != 0) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion tail did not converge");
     }
     if (rounds) *rounds = r;
     int nt = c.get(C_NTRI);
@@ -733,17 +885,20 @@ int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int n
 
 // ================================================================== flip rounds
 
-__global__ void k_flip_begin(int* cnt)
+// Flip round prologue: the previous round's output list becomes the input.
+__global__ void k_flip_begin(int* cnt, int move)
 {
+    if (move) cnt[C_NACT] = cnt[C_NACT2];
     cnt[C_NACT2] = 0;
     cnt[C_NFLIP] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_flip_test(Flip f, const int* list, const int* cnt, unsigned epoch)
+__global__ __launch_bounds__(kBlock) void k_flip_test(Flip f, const int* list, const int* cnt, unsigned epoch,
+                                                     int single)
 {
     int n = *cnt;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        flip_test(f, list ? list[i] : (int)i, epoch, list == nullptr);
+        flip_test(f, list ? list[i] : (int)i, epoch, list == nullptr, single != 0);
 }
 
 constexpr int FA_ITEMS = 4;  // list items per thread per block-wide append
@@ -764,7 +919,7 @@ __global__ __launch_bounds__(kBlock) void k_flip_apply(Flip f, const int* list, 
             if (i < n) {
                 int t = list ? list[i] : (int)i;
                 int nn = -1;
-                int res = flip_apply(f, t, epoch, &nn);
+                int res = flip_apply<false>(f, t, epoch, &nn);
                 if (res == 1 || res == 2) qa[j] = t;
                 if (res == 2) qb[j] = nn;
             }
@@ -805,7 +960,7 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(Flip f, int* A, int* B, i
     unsigned ep = epoch0;
     int r = 0, err = 0, flips = 0;
     while (n > 0 && r < max_rounds) {
-        for (int i = threadIdx.x; i < n; i += TAIL_BS) flip_test(f, A[i], ep, false);
+        for (int i = threadIdx.x; i < n; i += TAIL_BS) flip_test(f, A[i], ep, false, false);
         if (threadIdx.x == 0) {
             s_n2 = 0;
             s_nf = 0;
@@ -879,11 +1034,11 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
     f.cand = cand.p;
     f.mark = mark.p;
     f.err = c.dcnt.p + C_ERR;
-    int nact = init ? ninit : T;
+    // round 1 reads `init` (or the implicit list of all T triangles, counted
+    // in C_CONST); later rounds alternate between la and lb, counted in C_NACT
     const int* act = init;
     int* nxt = la.p;
     int* other = lb.p;
-    // implicit list: the count lives in C_CONST; explicit: in C_NACT
     if (init) {
         c.set(C_NACT, ninit);
         k_mark_list<<<grid_for(ninit), kBlock, 0, c.st>>>(init, ninit, mark.p, (int)c.epoch + 1);
@@ -891,68 +1046,101 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
     } else {
         c.set(C_CONST, T);
     }
-    int cnt_idx = init ? C_NACT : C_CONST;
     long long flips = 0;
     int r = 0;
-    while (nact > 0) {
-        if (nact <= c.tail_threshold) {
-            unsigned ep0 = c.epoch + 1;
-            if (!act) {  // small mesh: materialise the implicit list
-                k_iota<<<grid_for(nact), kBlock, 0, c.st>>>(lb.p, nact);
-                GCDT_LAUNCH_CHECK();
-                fill_i32(c, mark.p, (int)ep0, nact);
-                act = lb.p;
-                nxt = la.p;
-                c.set(C_NACT, nact);
-            }
-            {
-                ProfScope ps("flip_tail", c.st, nact);
-                k_flip_tail<<<1, TAIL_BS, 0, c.st>>>(f, (int*)act, nxt, lf.p, c.dcnt.p, ep0, 1 << 20);
-                GCDT_LAUNCH_CHECK();
-            }
-            c.read(0, C_NUM);
-            if (c.hcnt[C_ERR]) c.check_err();
-            if (c.hcnt[C_TAIL_ERR]) throw Error(GCDT_ERR_INTERNAL, "flip round without progress (tail)");
-            r += c.hcnt[C_TAIL_R];
-            c.epoch += (unsigned)c.hcnt[C_TAIL_R];
-            flips += c.hcnt[C_TAIL_F];
-            if (c.hcnt[C_TAIL_N] != 0) throw Error(GCDT_ERR_NOT_CONVERGED, "flip tail did not converge");
-            break;
-        }
+    int est = init ? ninit : T;
+    bool done = est == 0;
+    auto check_round = [&](const int* h) {
+        if (h[C_ERR]) c.check_err();
+        // the highest-keyed proposal always wins both of its triangles
+        if (h[C_NACT2] > 0 && h[C_NFLIP] == 0) throw Error(GCDT_ERR_INTERNAL, "flip round without progress");
+        flips += h[C_NFLIP] / 2;
+    };
+    while (!done && est > c.tail_threshold) {
         if (++r > 1000000) throw Error(GCDT_ERR_NOT_CONVERGED, "flip rounds did not converge");
         unsigned ep = c.next_epoch();
-        k_flip_begin<<<1, 1, 0, c.st>>>(c.dcnt.p);
+        int cnt_idx = (r == 1 && !init) ? C_CONST : C_NACT;
+        k_flip_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, r > 1);
         GCDT_LAUNCH_CHECK();
-        int g = grid_for(nact);
+        int g = grid_for(est);
         {
-            ProfScope ps("flip_test", c.st, nact);
-            k_flip_test<<<g, kBlock, 0, c.st>>>(f, act, c.dcnt.p + cnt_idx, ep);
+            ProfScope ps("flip_test", c.st, est);
+            k_flip_test<<<g, kBlock, 0, c.st>>>(f, act, c.dcnt.p + cnt_idx, ep, c.single_tester);
             GCDT_LAUNCH_CHECK();
         }
         {
-            ProfScope ps("flip_apply", c.st, nact);
-            k_flip_apply<<<grid_for((nact + FA_ITEMS - 1) / FA_ITEMS), kBlock, 0, c.st>>>(
+            ProfScope ps("flip_apply", c.st, est);
+            k_flip_apply<<<grid_for((est + FA_ITEMS - 1) / FA_ITEMS), kBlock, 0, c.st>>>(
                 f, act, c.dcnt.p + cnt_idx, ep, nxt, c.dcnt.p + C_NACT2, lf.p, c.dcnt.p + C_NFLIP);
             GCDT_LAUNCH_CHECK();
         }
         {
             ProfScope ps("flip_fix", c.st, 0);
-            k_flip_fix<<<grid_for(nact), kBlock, 0, c.st>>>(f, lf.p, c.dcnt.p + C_NFLIP, ep);
+            k_flip_fix<<<grid_for(est), kBlock, 0, c.st>>>(f, lf.p, c.dcnt.p + C_NFLIP, ep);
             GCDT_LAUNCH_CHECK();
         }
-        c.read(0, 8);
-        if (c.hcnt[C_ERR]) c.check_err();
-        // the highest-keyed proposal always wins both of its triangles
-        if (c.hcnt[C_NACT2] > 0 && c.hcnt[C_NFLIP] == 0) throw Error(GCDT_ERR_INTERNAL, "flip round without progress");
-        nact = c.hcnt[C_NACT2];
-        flips += c.hcnt[C_NFLIP] / 2;
-        c.set(C_NACT, nact);
+        c.snapshot(r & 1);
         act = nxt;
-        cnt_idx = C_NACT;
         std::swap(nxt, other);
+        if (r >= 2) {
+            const int* h = c.wait_snapshot((r - 1) & 1);  // round r-1 (round r is queued)
+            check_round(h);
+            if (h[C_NACT2] == 0) done = true;
+            est = h[C_NACT2];
+        }
+    }
+    if (r >= 1) {
+        const int* h = c.wait_snapshot(r & 1);
+        check_round(h);
+        if (h[C_NACT2] == 0) done = true;
+        est = h[C_NACT2];
+    }
+    if (!done) {
+        // the rest in one persistent workgroup
+        unsigned ep0 = c.epoch + 1;
+        if (!act) {  // small mesh: materialise the implicit list
+            k_iota<<<grid_for(est), kBlock, 0, c.st>>>(lb.p, est);
+            GCDT_LAUNCH_CHECK();
+            fill_i32(c, mark.p, (int)ep0, est);
+            act = lb.p;
+            nxt = la.p;
+            c.set(C_NACT, est);
+        } else if (r >= 1) {
+            k_flip_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, 1);
+            GCDT_LAUNCH_CHECK();
+        }
+        {
+            ProfScope ps("flip_tail", c.st, est);
+            k_flip_tail<<<1, TAIL_BS, 0, c.st>>>(f, (int*)act, nxt, lf.p, c.dcnt.p, ep0, 1 << 20);
+            GCDT_LAUNCH_CHECK();
+        }
+        c.read(0, C_NUM);
+        if (c.hcnt[C_ERR]) c.check_err();
+        if (c.hcnt[C_TAIL_ERR]) throw Error(GCDT_ERR_INTERNAL, "flip round without progress (tail)");
+        r += c.hcnt[C_TAIL_R];
+        c.epoch += (unsigned)c.hcnt[C_TAIL_R];
+        flips += c.hcnt[C_TAIL_F];
+        if (c.hcnt[C_TAIL_N] != 0) throw Error(GCDT_ERR_NOT_CONVERGED, "flip tail did not converge");
     }
     if (rounds) *rounds = r;
     return flips;
+}
+
+// ================================================================== flags -> list
+
+__global__ __launch_bounds__(kBlock) void k_flag_scatter(const int* flags, const int* off, int n, int* list)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        if (flags[i]) list[off[i]] = (int)i;
+}
+
+int flags_to_list(Ctx& c, const int* flags, int n, int* list)
+{
+    DBuf<int> off((size_t)n + 1);
+    scan_excl(c, flags, off.p, n);
+    k_flag_scatter<<<grid_for(n), kBlock, 0, c.st>>>(flags, off.p, n, list);
+    GCDT_LAUNCH_CHECK();
+    return c.get(C_TOTAL);
 }
 
 // ================================================================== leaves
@@ -982,6 +1170,65 @@ __global__ __launch_bounds__(kBlock) void k_leaf_scatter(const int3* tri, const 
             adj2[o] = a;
         }
     }
+}
+
+// Leaves in Morton order of their centroid's 1024^2 cell (the reference's
+// reorderFaces, kernel.cu:1886-1964, done as a counting sort fused with the
+// compaction): neighbouring triangles end up near each other in memory.
+__global__ __launch_bounds__(kBlock) void k_leaf_cells(const int2* pts, const int3* tri, const int* sons, int T, int* cnt,
+                                                       int* cellof)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        if (sons[i] != -1) continue;
+        int3 t = tri[i];
+        int2 a = pts[t.x], b = pts[t.y], c = pts[t.z];
+        int2 g = make_int2((int)(((long long)a.x + b.x + c.x) / 3), (int)(((long long)a.y + b.y + c.y) / 3));
+        int cell = (int)morton_cell(g);
+        cellof[i] = cell;
+        atomicAdd(&cnt[cell], 1);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaf_dest(const int* sons, const int* cellof, int T, int* cursor, int* dest)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x)
+        dest[i] = sons[i] == -1 ? atomicAdd(&cursor[cellof[i]], 1) : -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaf_move(const int3* tri, const int3* adj, const int* dest, int T, int3* tri2,
+                                                      int3* adj2, long long cap, int* err)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        int o = dest[i];
+        if (o < 0) continue;
+        if (o >= cap) {
+            atomicOr(err, MERR_CAPACITY);
+            continue;
+        }
+        tri2[o] = tri[i];
+        int3 a = adj[i];
+        a.x = a.x >= 0 ? dest[a.x] : -1;
+        a.y = a.y >= 0 ? dest[a.y] : -1;
+        a.z = a.z >= 0 ? dest[a.z] : -1;
+        adj2[o] = a;
+    }
+}
+
+int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2,
+                         int3* adj2, long long cap)
+{
+    DBuf<int> cnt(BITMAP_CELLS + 1), cellof(T > 0 ? T : 1), dest(T > 0 ? T : 1);
+    fill_i32(c, cnt.p, 0, BITMAP_CELLS + 1);
+    k_leaf_cells<<<grid_for(T), kBlock, 0, c.st>>>(pts, tri, sons, T, cnt.p, cellof.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl(c, cnt.p, cnt.p, BITMAP_CELLS);
+    int n = c.get(C_TOTAL);
+    k_leaf_dest<<<grid_for(T), kBlock, 0, c.st>>>(sons, cellof.p, T, cnt.p, dest.p);
+    GCDT_LAUNCH_CHECK();
+    k_leaf_move<<<grid_for(T), kBlock, 0, c.st>>>(tri, adj, dest.p, T, tri2, adj2, cap, c.dcnt.p + C_ERR);
+    GCDT_LAUNCH_CHECK();
+    c.check_err();
+    return n;
 }
 
 int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2, int3* adj2,

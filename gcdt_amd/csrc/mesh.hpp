@@ -93,6 +93,10 @@ struct Ctx {
     size_t ext_scratch_bytes = 0;
     // lists at or below this length finish in one persistent workgroup
     int tail_threshold = 4096;
+    // tunables (environment overrides for experiments): insertion pivot policy
+    // and whether both-active edges are tested by the lower triangle only
+    int pivot = 0;
+    int single_tester = 1;
     explicit Ctx(hipStream_t s);
     ~Ctx();
     Ctx(const Ctx&) = delete;
@@ -106,6 +110,11 @@ struct Ctx {
         return hcnt[i];
     }
     void set(int i, int v);
+    // pipelined snapshots of counters [0, 8): ring of 2 pinned slots + events
+    int* hring = nullptr;
+    hipEvent_t ring_ev[2] = {nullptr, nullptr};
+    void snapshot(int slot);          // async copy of counters 0..7 + event
+    const int* wait_snapshot(int slot);  // waits for that copy
     void zero_counters();
     void check_err();  // throws on a device error flag
 };
@@ -146,9 +155,19 @@ int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int n
                   int* rounds);
 // flip to (constrained) Delaunay; init list (nullptr = all T triangles)
 long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds);
+// spatial order: points sorted by the Morton code of their 1024^2 cell
+// (out[k] = pts[perm[k]], inv = perm^-1)
+void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv);
+void relabel_pairs(Ctx& c, const int2* in, int C, const int* inv, int n, int2* out);
+void unmap_tris(Ctx& c, int3* tri, int T, const int* perm, int n);
+// indices i with flags[i] != 0 (flags are 0/1), ascending; returns the count
+int flags_to_list(Ctx& c, const int* flags, int n, int* list);
 // leaves of a DAG -> compact arrays (adjacency remapped); returns count
 int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2, int3* adj2,
                   long long cap);
+// same, with the leaves placed in Morton order of their centroid cell
+int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2,
+                         int3* adj2, long long cap);
 // bitmap CSR over 1024^2 cells; allocates offsets/tris from the pool
 void build_bitmap(Ctx& c, const int2* pts, const int3* tri, int T, int** offsets, int** tris, int* ntris);
 // mark vertices of tri[0..T) in isv (must be zeroed), then locate the other

@@ -36,6 +36,14 @@ __device__ inline unsigned long long ld_u64(const unsigned long long* p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// COH: inside a persistent workgroup (atomic load); otherwise a kernel
+// boundary separates the claims from this read and a plain load suffices.
+template <bool COH>
+__host__ __device__ inline unsigned long long ld_claim(const unsigned long long* p)
+{
+    if (COH) return ld_u64(p);
+    return *p;
+}
 
 enum : int {
     MERR_CAPACITY = 1,     // triangle slots exhausted
@@ -65,7 +73,9 @@ struct Ins {
     Mesh m;
     int* loc;
     unsigned long long* claim;
+    unsigned long long* pkey;  // per point: its claim key of the current round
     int cap;
+    int pivot;                 // 0: random among the triangle's points, 1: central ones first
     int* err;
 };
 
@@ -95,31 +105,53 @@ __host__ __device__ inline unsigned long long ins_key(int p, int onedge, unsigne
 
 __host__ __device__ inline unsigned long long ins_key_of(const Ins& s, int p, int t, int k, unsigned epoch)
 {
-    int3 tv = s.m.tri[t];
-    unsigned close = closeness_rank(s.m.pts[tv.x], s.m.pts[tv.y], s.m.pts[tv.z], s.m.pts[p], p);
+    unsigned close;
+    if (s.pivot == 1) {
+        int3 tv = s.m.tri[t];
+        close = closeness_rank(s.m.pts[tv.x], s.m.pts[tv.y], s.m.pts[tv.z], s.m.pts[p], p);
+    } else {
+        close = hash32((uint32_t)p) & 2047u;
+    }
     return ins_key(p, k < 3, close, epoch);
+}
+
+// Computes p's key for this round (cached in pkey) and returns its triangle
+// (the caller performs the claim so a wave can combine claims on one triangle);
+// -1 when p is not pending.  *t2 receives the second triangle of an on-edge
+// point, else -1.
+__host__ __device__ inline int ins_key_round(const Ins& s, int p, unsigned epoch, unsigned long long* key, int* t2)
+{
+    int l = s.loc[p];
+    *t2 = -1;
+    if (l < 0) return -1;
+    int t = l >> 2, k = l & 3;
+    *key = ins_key_of(s, p, t, k, epoch);
+    s.pkey[p] = *key;
+    if (k < 3) *t2 = A3(s.m, t, k);
+    return t;
 }
 
 __host__ __device__ inline void ins_claim(const Ins& s, int p, unsigned epoch)
 {
-    int l = s.loc[p];
-    if (l < 0) return;
-    int t = l >> 2, k = l & 3;
-    unsigned long long key = ins_key_of(s, p, t, k, epoch);
+    unsigned long long key;
+    int t2;
+    int t = ins_key_round(s, p, epoch, &key, &t2);
+    if (t < 0) return;
     at_max(&s.claim[t], key);
-    if (k < 3) at_max(&s.claim[A3(s.m, t, k)], key);
+    if (t2 >= 0) at_max(&s.claim[t2], key);
 }
 
 // slots needed by p if it won every triangle it claimed (3 interior, 4 edge)
+template <bool COH = true>
 __host__ __device__ inline int ins_need(const Ins& s, int p, unsigned epoch)
 {
     int l = s.loc[p];
     if (l < 0) return 0;
     int t = l >> 2, k = l & 3;
-    unsigned long long key = ins_key_of(s, p, t, k, epoch);
-    if (ld_u64(&s.claim[t]) != key) return 0;
+    unsigned long long key = s.pkey[p];
+    if (ld_claim<COH>(&s.claim[t]) != key) return 0;
     if (k == 3) return 3;
-    return ld_u64(&s.claim[A3(s.m, t, k)]) == key ? 4 : 0;
+    return ld_claim<COH>(&s.claim[A3(s.m, t, k)]) == key ? 4 : 0;
 }
 
 // Split the triangle(s) of winner p into new slots [base, base+need).
@@ -272,7 +304,7 @@ __host__ __device__ inline int opposite_vertex(const Mesh& m, int n, int u, int 
 // (all_active: every triangle is active this epoch).  Every illegal edge keeps
 // an active triangle: the lower one proposes an edge, so it is re-queued or
 // flipped.
-__host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch, bool all_active)
+__host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch, bool all_active, bool single = true)
 {
     const Mesh& m = f.m;
     int3 tv = m.tri[t];
@@ -284,7 +316,7 @@ __host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch, 
         if ((cf >> k) & 1) continue;
         int n = get3(ta, k);
         if (n < 0) continue;
-        if (n < t && (all_active || f.mark[n] == (int)epoch)) continue;
+        if (single && n < t && (all_active || f.mark[n] == (int)epoch)) continue;
         int o = opposite_vertex(m, n, get3(tv, nx3(k)), get3(tv, pv3(k)));
         if (incircle(A, B, C, m.pts[o]) > 0) {
             cand = k;
@@ -302,6 +334,7 @@ __host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch, 
 // proposal that wins and flips t (its winner re-queues t, so t does not).
 // t=(p,u,v), n=(o,v,u)  ->  t=(p,u,o), n=(p,o,v); outer edges are marked
 // for flip_fix.  Reference: flipEdgeFE / multiFlipFE, kernel.cu:2681-2710,3548.
+template <bool COH = true>
 __host__ __device__ inline int flip_apply(const Flip& f, int t, unsigned epoch, int* nout)
 {
     const Mesh& m = f.m;
@@ -311,13 +344,13 @@ __host__ __device__ inline int flip_apply(const Flip& f, int t, unsigned epoch, 
     int3 ta = m.adj[t];
     int n = get3(ta, k);
     unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
-    unsigned long long ct = ld_u64(&f.claim[t]);
-    if (ct != key || ld_u64(&f.claim[n]) != key) {
+    unsigned long long ct = ld_claim<COH>(&f.claim[t]);
+    if (ct != key || ld_claim<COH>(&f.claim[n]) != key) {
         if (ct != key) {
             // ct was proposed by triangle t2 for the edge it shares with t; t2
             // wins (and flips t) iff it also holds its own claim
             int t2 = (int)((ct & 0xffffffffu) / 3);
-            if ((ct >> 32) == epoch && ld_u64(&f.claim[t2]) == ct) return 3;
+            if ((ct >> 32) == epoch && ld_claim<COH>(&f.claim[t2]) == ct) return 3;
         }
         return 1;
     }
