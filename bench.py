@@ -63,6 +63,12 @@ def main(argv=None):
     ap.add_argument("--circles", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096, help="instances in the batched workload (0 = skip)")
+    ap.add_argument("--per-instance", type=int, default=256,
+                    help="instances timed with one plain solve each, for the batched comparison")
+    ap.add_argument("--serve-count", type=int, default=128, help="requests in the serving run (0 = skip)")
+    ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving run")
+    ap.add_argument("--serve-workers", type=int, default=3, help="worker streams of the serving run")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every GPU box here has one MI355X, and this CDT path has no "
@@ -156,6 +162,9 @@ def main(argv=None):
                "sample": f"the full workload ({n} points, {c} segments), serial exact-predicate CDT, "
                          f"one host core, {dtc:.2f} s"}
 
+    batched = bench_batched(args) if args.batch_count > 0 else None
+    serving = bench_serving(args) if args.serve_count > 0 else None
+
     out = {
         "metric": "Mpts/s CDT at 10M pts; batched CDTs/s vs per-instance launch; p99 latency",
         "value": round(mpts, 4),
@@ -178,10 +187,123 @@ def main(argv=None):
         "validity": valid,
         "last_step": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in steps[-1].items()},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     for b in (dp, dc, dt, da):
         b.free()
+
+
+def bench_batched(args):
+    """Batched workload: `batch_count` star-polygon instances (point counts
+    log-uniform in [64, 65536], 0-4 holes) solved in one gcdt_solve_batch,
+    against one plain gcdt_solve per instance on the first `per_instance`."""
+    import numpy as np
+
+    from gcdt_amd import DeviceBuffer, gen, pack_instances, solve_batch_device, solve_device
+    from gcdt_amd.lib import check, lib
+
+    L = lib()
+    t0 = time.time()
+    raw = gen.batched_instances(count=args.batch_count)
+    inst = [(gen.to_grid(p), c) for p, c in raw]
+    flat_p, poff, flat_c, coff, toff = pack_instances(inst)
+    print(f"# batched workload: {len(inst)} instances, {len(flat_p)} points, {len(flat_c)} segments "
+          f"(generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    dp = DeviceBuffer.from_array(flat_p)
+    dc = DeviceBuffer.from_array(flat_c)
+    dt = DeviceBuffer(12 * int(toff[-1]))
+    st = solve_batch_device(dp, poff, dc, coff, dt)  # warmup
+    reps = 2
+    check(L.gcdt_synchronize(), "sync")
+    tb = time.perf_counter()
+    for _ in range(reps):
+        st = solve_batch_device(dp, poff, dc, coff, dt)
+    check(L.gcdt_synchronize(), "sync")
+    ms_batch = (time.perf_counter() - tb) * 1000.0 / reps
+    print(f"# batched solve: {ms_batch:.1f} ms ({st})", file=sys.stderr, flush=True)
+    # exactness of a sample of instances
+    from oracle import oracle as O
+    allt = dt.to_array(np.int32, (int(toff[-1]), 3))
+    checked = 0
+    bad = 0
+    for i in range(0, len(inst), max(1, len(inst) // 32)):
+        g, c = inst[i]
+        v = O.validate(g, allt[toff[i]:toff[i + 1]], c)
+        checked += 1
+        bad += 0 if v["valid"] else 1
+    for b in (dp, dc, dt):
+        b.free()
+    # one plain solve per instance
+    m = min(args.per_instance, len(inst))
+    bufs = []
+    for g, c in inst[:m]:
+        bufs.append((DeviceBuffer.from_array(g), DeviceBuffer.from_array(c) if len(c) else None,
+                     DeviceBuffer(12 * (2 * len(g) + 2)), DeviceBuffer(12 * (2 * len(g) + 2)), len(g), len(c)))
+    solve_device(bufs[0][0], bufs[0][4], bufs[0][1], bufs[0][5], bufs[0][2], bufs[0][3])  # warmup
+    check(L.gcdt_synchronize(), "sync")
+    ts = time.perf_counter()
+    for dpi, dci, dti, dai, n, c in bufs:
+        solve_device(dpi, n, dci, c, dti, dai)
+    check(L.gcdt_synchronize(), "sync")
+    ms_single = (time.perf_counter() - ts) * 1000.0
+    pts_single = sum(b[4] for b in bufs)
+    for b in bufs:
+        for x in b[:4]:
+            if x is not None:
+                x.free()
+    batched_cdts = len(inst) / (ms_batch / 1000.0)
+    single_cdts = m / (ms_single / 1000.0)
+    # the per-instance rate is measured on the first m instances; scale by point
+    # counts too so an unrepresentative subset is visible
+    return {"instances": len(inst), "points": int(len(flat_p)), "segments": int(len(flat_c)),
+            "ms_batched": round(ms_batch, 3), "batched_cdts_per_s": round(batched_cdts, 1),
+            "batched_mpts_per_s": round(len(flat_p) / (ms_batch / 1000.0) / 1e6, 3),
+            "per_instance_subset": m, "per_instance_ms": round(ms_single, 3),
+            "per_instance_cdts_per_s": round(single_cdts, 1),
+            "per_instance_mpts_per_s": round(pts_single / (ms_single / 1000.0) / 1e6, 3),
+            "speedup_cdts": round(batched_cdts / single_cdts, 2),
+            "rounds": {"insert": st["insert_rounds"], "flip": st["flip_rounds"],
+                       "constraints": st["constraint_rounds"]},
+            "validated_instances": checked, "invalid_instances": bad}
+
+
+def bench_serving(args):
+    """Serving workload: `serve_count` requests (log-uniform 1e3..1e6 points,
+    polygon boundaries of 1-5% of the points) arriving as a Poisson process at
+    `serve_qps`; worker streams merge small requests into batched solves.
+    Reports throughput and p50/p99 latency from arrival to result on device."""
+    from gcdt_amd import gen
+    from gcdt_amd import serve as S
+
+    t0 = time.time()
+    raw = gen.serving_requests(count=args.serve_count, qps=args.serve_qps)
+    reqs = [S.Request(i, t, gen.to_grid(p), c) for i, (t, p, c) in enumerate(raw)]
+    npts = sum(r.n for r in reqs)
+    print(f"# serving workload: {len(reqs)} requests, {npts} points (generated in {time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    # warm the pool and the code objects on a few requests, outside the timed run
+    S.Server(workers=args.serve_workers).run([S.Request(-1 - i, 0.0, r.pts, r.cons) for i, r in
+                                              enumerate(reqs[:4])], realtime=False)
+    srv = S.Server(workers=args.serve_workers)
+    st = srv.run(reqs)
+    summ = st.summary(args.serve_qps)
+    from oracle import oracle as O
+    bad = 0
+    checked = 0
+    for r in reqs[:: max(1, len(reqs) // 16)]:
+        v = O.validate(r.pts, r.fetch(), r.cons)
+        checked += 1
+        bad += 0 if v["valid"] else 1
+    for r in reqs:
+        r.fetch()
+    summ.update({"points": int(npts), "workers": args.serve_workers, "small_n": srv.small_n,
+                 "mpts_per_s": round(npts / max(st.makespan_s, 1e-9) / 1e6, 3),
+                 "validated_requests": checked, "invalid_requests": bad,
+                 "latency": "arrival to triangles resident on the device"})
+    print(f"# serving: {summ}", file=sys.stderr, flush=True)
+    return summ
 
 
 if __name__ == "__main__":

@@ -9,7 +9,7 @@ import ctypes
 
 import numpy as np
 
-from .lib import Stats, check, lib
+from .lib import BatchStats, Stats, check, lib
 
 
 class DeviceBuffer:
@@ -82,3 +82,61 @@ def solve(pts_grid: np.ndarray, cons: np.ndarray | None = None, with_adj: bool =
             if b is not None:
                 b.free()
     return (tris, st, adj) if with_adj else (tris, st)
+
+
+def pack_instances(instances):
+    """[(pts_grid (n_i,2), cons (c_i,2) local ids)] -> flat arrays + offsets."""
+    pts = [np.ascontiguousarray(p, dtype=np.int32).reshape(-1, 2) for p, _ in instances]
+    cons = [np.ascontiguousarray(c if c is not None else np.zeros((0, 2)), dtype=np.int32).reshape(-1, 2)
+            for _, c in instances]
+    poff = np.zeros(len(instances) + 1, np.int64)
+    coff = np.zeros(len(instances) + 1, np.int64)
+    poff[1:] = np.cumsum([len(p) for p in pts])
+    coff[1:] = np.cumsum([len(c) for c in cons])
+    flat_p = np.concatenate(pts) if pts else np.zeros((0, 2), np.int32)
+    flat_c = np.concatenate(cons) if cons else np.zeros((0, 2), np.int32)
+    toff = np.zeros(len(instances) + 1, np.int64)
+    toff[1:] = np.cumsum([2 * len(p) + 2 for p in pts])
+    return flat_p, poff, flat_c, coff, toff
+
+
+def solve_batch_device(d_pts: DeviceBuffer, poff: np.ndarray, d_cons, coff: np.ndarray, d_tris: DeviceBuffer,
+                       stream=None):
+    K = len(poff) - 1
+    st = BatchStats()
+    poff = np.ascontiguousarray(poff, dtype=np.int64)
+    coff = np.ascontiguousarray(coff, dtype=np.int64)
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    rc = lib().gcdt_solve_batch(d_pts.ptr, poff.ctypes.data_as(P64), K,
+                                d_cons.ptr if d_cons is not None else None,
+                                coff.ctypes.data_as(P64) if d_cons is not None else None,
+                                d_tris.ptr, ctypes.byref(st), stream)
+    check(rc, "gcdt_solve_batch")
+    return st.as_dict()
+
+
+def solve_batch(instances):
+    """K independent CDTs in one batched solve.  Returns (list of per-instance
+    triangle arrays with local vertex ids, stats)."""
+    flat_p, poff, flat_c, coff, toff = pack_instances(instances)
+    dp = DeviceBuffer.from_array(flat_p)
+    dc = DeviceBuffer.from_array(flat_c) if len(flat_c) else None
+    dt = DeviceBuffer(12 * int(toff[-1]))
+    try:
+        st = solve_batch_device(dp, poff, dc, coff, dt)
+        allt = dt.to_array(np.int32, (int(toff[-1]), 3))
+    finally:
+        for b in (dp, dc, dt):
+            if b is not None:
+                b.free()
+    return [allt[toff[i]:toff[i + 1]] for i in range(len(instances))], st
+
+
+def set_tunable(name: str, value: int):
+    check(lib().gcdt_set_tunable(name.encode(), int(value)), "gcdt_set_tunable")
+
+
+def get_tunable(name: str) -> int:
+    v = ctypes.c_int(0)
+    check(lib().gcdt_get_tunable(name.encode(), ctypes.byref(v)), "gcdt_get_tunable")
+    return v.value

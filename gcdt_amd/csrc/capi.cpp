@@ -6,6 +6,7 @@
 #include <chrono>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "mesh.hpp"
 
@@ -147,6 +148,49 @@ int gcdt_memset(void* dst, int value, size_t bytes)
 
 size_t gcdt_scratch_bytes(int64_t n) { return sizeof(int) * (size_t)scan_scratch_ints(n); }
 
+int gcdt_stream_create(void** stream)
+{
+    API_BEGIN
+    if (!stream) return fail(GCDT_ERR_INVALID, "null stream pointer");
+    hipStream_t s = nullptr;
+    GCDT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_stream_destroy(void* stream)
+{
+    API_BEGIN
+    if (stream) GCDT_HIP(hipStreamDestroy((hipStream_t)stream));
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_stream_synchronize(void* stream)
+{
+    API_BEGIN
+    GCDT_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_memcpy_h2d_async(void* dst, const void* src, size_t bytes, void* stream)
+{
+    API_BEGIN
+    GCDT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_memcpy_d2h_async(void* dst, const void* src, size_t bytes, void* stream)
+{
+    API_BEGIN
+    GCDT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    return GCDT_OK;
+    API_END
+}
+
 int gcdt_prof_enable(int on)
 {
     prof_enable(on != 0);
@@ -283,7 +327,7 @@ static void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, i
         fill_i32(c, cflag.p, 0, T);
         fill_i32(c, touched.p, 0, T);
         Mesh m{pts, tris_out, adj_out, nullptr, cflag.p};
-        ConsResult cr = insert_constraints(c, m, T, N + 4, cons, C, touched.p, nullptr);
+        ConsResult cr = insert_constraints(c, m, T, N + 4, N, cons, C, touched.p, nullptr);
         st.constraint_rounds = cr.rounds;
         st.segments = cr.pieces;
         st.splits = cr.pieces - C;
@@ -592,7 +636,7 @@ int gcdt_delaunay_new(void* tmp, size_t tmpBytes, gcdt_int2** points, gcdt_int3*
     fill_i32(c, cflag.p, 0, T);
     build_adjacency(c, (const int3*)*tris, T, numPoints, adj.p);
     Mesh m{(const int2*)*points, (int3*)*tris, adj.p, nullptr, cflag.p};
-    mark_constraint_chains(c, m, T, numPoints, (const int2*)cons, numCons);
+    mark_constraint_chains(c, m, T, numPoints, numPoints - 4, (const int2*)cons, numCons);
     int r = 0;
     flip_rounds(c, m, T, nullptr, 0, &r);
     if (ms) *ms = t.ms();
@@ -628,7 +672,7 @@ int gcdt_add_constraints(void* tmp, size_t tmpBytes, const gcdt_int2* points, gc
     fill_i32(c, cflag.p, 0, T);
     fill_i32(c, touched.p, 0, T);
     Mesh m{(const int2*)points, tri.p, adj.p, nullptr, cflag.p};
-    insert_constraints(c, m, T, numPoints + 4, (const int2*)cons, numCons, touched.p, nullptr);
+    insert_constraints(c, m, T, numPoints + 4, numPoints, (const int2*)cons, numCons, touched.p, nullptr);
     // crossed triangles are retired in the DAG; their replacements go out
     k_retire<<<grid_for(T), kBlock, 0, c.st>>>(touched.p, slot.p, T, sons);
     GCDT_LAUNCH_CHECK();
@@ -647,3 +691,131 @@ int gcdt_add_constraints(void* tmp, size_t tmpBytes, const gcdt_int2* points, gc
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ batches
+
+extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int K, const gcdt_int2* cons,
+                                const int64_t* cons_offsets, gcdt_int3* tris_out, gcdt_batch_stats* stats,
+                                void* stream)
+{
+    API_BEGIN
+    if (K <= 0 || !point_offsets || !tris_out) return fail(GCDT_ERR_INVALID, "bad arguments");
+    if (point_offsets[0] != 0) return fail(GCDT_ERR_INVALID, "point_offsets[0] must be 0");
+    std::vector<long long> poff(K + 1), coff(K + 1, 0), toff(K + 1, 0);
+    for (int i = 0; i <= K; i++) {
+        poff[i] = point_offsets[i];
+        if (i && poff[i] < poff[i - 1]) return fail(GCDT_ERR_INVALID, "point_offsets not monotone");
+        if (cons_offsets) {
+            coff[i] = cons_offsets[i];
+            if (i && coff[i] < coff[i - 1]) return fail(GCDT_ERR_INVALID, "cons_offsets not monotone");
+        }
+        if (i) toff[i] = toff[i - 1] + 2 * (poff[i] - poff[i - 1]) + 2;
+    }
+    if (cons_offsets && coff[0] != 0) return fail(GCDT_ERR_INVALID, "cons_offsets[0] must be 0");
+    long long NP = poff[K], NC = coff[K];
+    if (NP > 0 && !points) return fail(GCDT_ERR_INVALID, "no points");
+    if (NC > 0 && !cons) return fail(GCDT_ERR_INVALID, "no constraints");
+    long long nv = NP + 4LL * K, cap = 2LL * K + 4 * NP + 8;
+    if (cap > (1LL << 30) || nv > (1LL << 29)) return fail(GCDT_ERR_INVALID, "batch too large");
+    hipStream_t st = (hipStream_t)stream;
+    Ctx c(st);
+    gcdt_batch_stats bs{};
+    Timer total(c.st);
+    DBuf<long long> d_poff(K + 1), d_coff(K + 1), d_toff(K + 1);
+    GCDT_HIP(hipMemcpyAsync(d_poff.p, poff.data(), 8 * (K + 1), hipMemcpyHostToDevice, st));
+    GCDT_HIP(hipMemcpyAsync(d_coff.p, coff.data(), 8 * (K + 1), hipMemcpyHostToDevice, st));
+    GCDT_HIP(hipMemcpyAsync(d_toff.p, toff.data(), 8 * (K + 1), hipMemcpyHostToDevice, st));
+    check_points(c, (const int2*)points, (int)NP);
+    DBuf<int2> pts(nv);
+    if (NP > 0) GCDT_HIP(hipMemcpyAsync(pts.p, points, 8 * NP, hipMemcpyDeviceToDevice, st));
+    DBuf<int> vinst(NP > 0 ? NP : 1);
+    batch_vinst(c, d_poff.p, K, NP, vinst.p);
+    // ---- insertion from every instance's two corner triangles
+    Timer t1(c.st);
+    DBuf<int3> tA(cap), aA(cap);
+    DBuf<int> sA(cap), loc(nv), pend(NP > 0 ? NP : 1), pend2(NP > 0 ? NP : 1);
+    DBuf<unsigned long long> clA(cap), pkey(nv);
+    fill_i32(c, sA.p, -1, cap);
+    fill_u64(c, clA.p, 0, cap);
+    batch_init(c, pts.p, tA.p, aA.p, K, NP);
+    batch_locate(c, pts.p, vinst.p, NP, loc.p, pend.p);
+    Ins s;
+    s.m = Mesh{pts.p, tA.p, aA.p, sA.p, nullptr};
+    s.loc = loc.p;
+    s.claim = clA.p;
+    s.pkey = pkey.p;
+    s.cap = (int)cap;
+    s.err = c.dcnt.p + C_ERR;
+    int ri = 0;
+    int nt = insert_rounds(c, s, pend, pend2, (int)NP, 2 * K, &ri);
+    clA.release();
+    pkey.release();
+    loc.release();
+    pend.release();
+    pend2.release();
+    long long Tn = 2 * NP + 2LL * K;
+    DBuf<int3> tri(Tn), adj(Tn);
+    int T = filter_leaves(c, tA.p, aA.p, sA.p, nt, tri.p, adj.p, Tn);
+    tA.release();
+    aA.release();
+    sA.release();
+    if (T != Tn) throw Error(GCDT_ERR_INTERNAL, "batch leaf count mismatch");
+    bs.ms_insert = t1.ms();
+    bs.insert_rounds = ri;
+    // ---- Delaunay flips of all instances together
+    Timer t2(c.st);
+    int fr = 0;
+    bs.flips += flip_rounds(c, Mesh{pts.p, tri.p, adj.p, nullptr, nullptr}, T, nullptr, 0, &fr);
+    bs.flip_rounds = fr;
+    bs.ms_flip = t2.ms();
+    // ---- constraints
+    Timer t3(c.st);
+    if (NC > 0) {
+        DBuf<int2> cg(NC);
+        batch_cons(c, (const int2*)cons, d_coff.p, d_poff.p, K, NC, cg.p);
+        DBuf<int> cflag(T), touched(T), tl(T);
+        fill_i32(c, cflag.p, 0, T);
+        fill_i32(c, touched.p, 0, T);
+        Mesh m{pts.p, tri.p, adj.p, nullptr, cflag.p};
+        ConsResult cr = insert_constraints(c, m, T, (int)nv, (int)NP, cg.p, (int)NC, touched.p, nullptr);
+        bs.constraint_rounds = cr.rounds;
+        bs.segments = cr.pieces;
+        int ntl = flags_to_list(c, touched.p, T, tl.p);
+        if (ntl > 0) {
+            int f3 = 0;
+            bs.flips += flip_rounds(c, m, T, tl.p, ntl, &f3);
+            bs.flip_rounds += f3;
+        }
+    }
+    bs.ms_constraints = t3.ms();
+    batch_out(c, tri.p, T, vinst.p, d_poff.p, d_toff.p, NP, K, (int3*)tris_out);
+    bs.ms_total = total.ms();
+    bs.num_instances = K;
+    bs.num_points = NP;
+    bs.num_triangles = T;
+    if (stats) *stats = bs;
+    return GCDT_OK;
+    API_END
+}
+
+extern "C" int gcdt_set_tunable(const char* name, int value)
+{
+    if (!name) return fail(GCDT_ERR_INVALID, "null name");
+    std::string n(name);
+    if (n == "tail_threshold") Tunables::get().tail_threshold = value;
+    else if (n == "pivot") Tunables::get().pivot = value;
+    else if (n == "single_tester") Tunables::get().single_tester = value;
+    else return fail(GCDT_ERR_INVALID, "unknown tunable " + n);
+    return GCDT_OK;
+}
+
+extern "C" int gcdt_get_tunable(const char* name, int* value)
+{
+    if (!name || !value) return fail(GCDT_ERR_INVALID, "null argument");
+    std::string n(name);
+    if (n == "tail_threshold") *value = Tunables::get().tail_threshold;
+    else if (n == "pivot") *value = Tunables::get().pivot;
+    else if (n == "single_tester") *value = Tunables::get().single_tester;
+    else return fail(GCDT_ERR_INVALID, "unknown tunable " + n);
+    return GCDT_OK;
+}

@@ -29,11 +29,11 @@
 
 namespace gcdt {
 
-__global__ __launch_bounds__(kBlock) void k_split_count(Cons c, const int2* cons, int C, int nv, int* cnt)
+__global__ __launch_bounds__(kBlock) void k_split_count(Cons c, const int2* cons, int C, int nmax, int* cnt)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < C; i += (long long)gridDim.x * blockDim.x) {
         int2 e = cons[i];
-        if (e.x < 0 || e.y < 0 || e.x >= nv - 4 || e.y >= nv - 4 || e.x == e.y) {
+        if (e.x < 0 || e.y < 0 || e.x >= nmax || e.y >= nmax || e.x == e.y) {
             atomicOr(c.err, MERR_INVALID);
             cnt[i] = 0;
             continue;
@@ -166,10 +166,10 @@ __global__ void k_cav_begin(int* cnt)
 
 // Split pass: pieces of every constraint (existing edges marked constrained
 // and stored with a negative first vertex).  Returns the piece count.
-static int split_pieces(Ctx& cx, const Cons& c, const int2* cons, int C, int nv, DBuf<int2>& pieces)
+static int split_pieces(Ctx& cx, const Cons& c, const int2* cons, int C, int nmax, DBuf<int2>& pieces)
 {
     DBuf<int> cnt((size_t)C + 1);
-    k_split_count<<<grid_for(C), kBlock, 0, cx.st>>>(c, cons, C, nv, cnt.p);
+    k_split_count<<<grid_for(C), kBlock, 0, cx.st>>>(c, cons, C, nmax, cnt.p);
     GCDT_LAUNCH_CHECK();
     cx.check_err();
     scan_excl_total(cx, cnt.p, cnt.p, C);
@@ -181,7 +181,7 @@ static int split_pieces(Ctx& cx, const Cons& c, const int2* cons, int C, int nv,
     return np;
 }
 
-long long mark_constraint_chains(Ctx& cx, const Mesh& m, int T, int nv, const int2* cons, int C)
+long long mark_constraint_chains(Ctx& cx, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C)
 {
     if (C <= 0) return 0;
     DBuf<int> vt(nv);
@@ -196,7 +196,7 @@ long long mark_constraint_chains(Ctx& cx, const Mesh& m, int T, int nv, const in
     c.cavof = nullptr;
     c.err = cx.dcnt.p + C_ERR;
     DBuf<int2> pieces;
-    return split_pieces(cx, c, cons, C, nv, pieces);
+    return split_pieces(cx, c, cons, C, nmax, pieces);
 }
 
 __global__ __launch_bounds__(kBlock) void k_touched(const int* stamp, int T, int* touched)
@@ -205,8 +205,8 @@ __global__ __launch_bounds__(kBlock) void k_touched(const int* stamp, int T, int
         touched[i] = stamp[i] != 0;
 }
 
-ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, const int2* cons, int C, int* touched,
-                              int* ntouched)
+ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C,
+                              int* touched, int* ntouched)
 {
     (void)ntouched;
     ConsResult res;
@@ -228,7 +228,7 @@ ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, const int2*
     c.err = cx.dcnt.p + C_ERR;
     // 1. split into pieces
     DBuf<int2> pieces;
-    int np = split_pieces(cx, c, cons, C, nv, pieces);
+    int np = split_pieces(cx, c, cons, C, nmax, pieces);
     res.pieces = np;
     // 2. rounds over pieces that need a cavity
     DBuf<int> qa(np > 0 ? np : 1), qb(np > 0 ? np : 1), ncross(np > 0 ? np : 1);

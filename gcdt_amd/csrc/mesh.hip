@@ -15,25 +15,83 @@ namespace gcdt {
 
 // ================================================================== context
 
+Tunables& Tunables::get()
+{
+    static Tunables t = [] {
+        Tunables x;
+        if (const char* e = getenv("GCDT_TAIL_THRESHOLD")) x.tail_threshold = atoi(e);
+        if (const char* e = getenv("GCDT_PIVOT")) x.pivot = atoi(e);
+        if (const char* e = getenv("GCDT_SINGLE_TESTER")) x.single_tester = atoi(e);
+        return x;
+    }();
+    return t;
+}
+
+thread_local std::vector<PendingFree>* g_defer = nullptr;
+
+namespace {
+// Pinned counter mirrors and ring events, reused across solves: allocating
+// and freeing pinned memory per call costs more than a small solve and
+// hipHostFree waits for the whole device, which would serialise solves that
+// run on different streams.
+struct HostSlot {
+    int* hcnt;
+    int* hring;
+    hipEvent_t ev[2];
+};
+std::mutex g_host_mu;
+std::vector<HostSlot> g_host_free;
+
+HostSlot take_host_slot()
+{
+    {
+        std::lock_guard<std::mutex> g(g_host_mu);
+        if (!g_host_free.empty()) {
+            HostSlot h = g_host_free.back();
+            g_host_free.pop_back();
+            return h;
+        }
+    }
+    HostSlot h{};
+    GCDT_HIP(hipHostMalloc((void**)&h.hcnt, sizeof(int) * C_NUM, hipHostMallocDefault));
+    GCDT_HIP(hipHostMalloc((void**)&h.hring, sizeof(int) * 16, hipHostMallocDefault));
+    GCDT_HIP(hipEventCreateWithFlags(&h.ev[0], hipEventDisableTiming));
+    GCDT_HIP(hipEventCreateWithFlags(&h.ev[1], hipEventDisableTiming));
+    return h;
+}
+}  // namespace
+
 Ctx::Ctx(hipStream_t s) : st(s)
 {
-    if (const char* e = getenv("GCDT_TAIL_THRESHOLD")) tail_threshold = atoi(e);
-    if (const char* e = getenv("GCDT_PIVOT")) pivot = atoi(e);
-    if (const char* e = getenv("GCDT_SINGLE_TESTER")) single_tester = atoi(e);
+    tail_threshold = Tunables::get().tail_threshold;
+    pivot = Tunables::get().pivot;
+    single_tester = Tunables::get().single_tester;
     dcnt.alloc(C_NUM);
-    GCDT_HIP(hipHostMalloc((void**)&hcnt, sizeof(int) * C_NUM, hipHostMallocDefault));
-    GCDT_HIP(hipHostMalloc((void**)&hring, sizeof(int) * 16, hipHostMallocDefault));
-    GCDT_HIP(hipEventCreateWithFlags(&ring_ev[0], hipEventDisableTiming));
-    GCDT_HIP(hipEventCreateWithFlags(&ring_ev[1], hipEventDisableTiming));
+    HostSlot h = take_host_slot();
+    hcnt = h.hcnt;
+    hring = h.hring;
+    ring_ev[0] = h.ev[0];
+    ring_ev[1] = h.ev[1];
+    if (st) {
+        prev_defer = g_defer;
+        g_defer = &deferred;
+    }
     zero_counters();
 }
 
 Ctx::~Ctx()
 {
-    if (hcnt) (void)hipHostFree(hcnt);
-    if (hring) (void)hipHostFree(hring);
-    for (auto& e : ring_ev)
-        if (e) (void)hipEventDestroy(e);
+    if (st) {
+        // every kernel of this solve has finished before its blocks go back
+        (void)hipStreamSynchronize(st);
+        g_defer = prev_defer;
+        for (auto& f : deferred) getMemCacheRef().releaseAbtrtMem(f.p, f.bytes);
+        deferred.clear();
+    }
+    if (hcnt) {
+        std::lock_guard<std::mutex> g(g_host_mu);
+        g_host_free.push_back(HostSlot{hcnt, hring, {ring_ev[0], ring_ev[1]}});
+    }
 }
 
 void Ctx::snapshot(int slot)
@@ -1124,6 +1182,125 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
     }
     if (rounds) *rounds = r;
     return flips;
+}
+
+// ================================================================== batches of instances
+
+// instance of each point (binary search over the point offsets)
+__global__ __launch_bounds__(kBlock) void k_batch_vinst(const long long* off, int K, long long NP, int* vinst)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
+        int lo = 0, hi = K;  // last instance with off[inst] <= i
+        while (lo + 1 < hi) {
+            int mid = (lo + hi) >> 1;
+            if (off[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        vinst[i] = lo;
+    }
+}
+
+// corners NP+4i..NP+4i+3 and the two triangles 2i, 2i+1 of every instance
+__global__ __launch_bounds__(kBlock) void k_batch_init(int2* pts, int3* tri, int3* adj, int K, long long NP)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < K; i += (long long)gridDim.x * blockDim.x) {
+        int c = (int)(NP + 4 * i);
+        pts[c] = make_int2(0, 0);
+        pts[c + 1] = make_int2(GRID_R, 0);
+        pts[c + 2] = make_int2(GRID_R, GRID_R);
+        pts[c + 3] = make_int2(0, GRID_R);
+        int t = (int)(2 * i);
+        tri[t] = make_int3(c, c + 1, c + 3);
+        tri[t + 1] = make_int3(c + 1, c + 2, c + 3);
+        adj[t] = make_int3(t + 1, -1, -1);
+        adj[t + 1] = make_int3(-1, t, -1);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_locate(const int2* pts, const int* vinst, long long NP, int* loc, int* pend)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
+        int t0 = 2 * vinst[i];
+        int o = orient2d(make_int2(GRID_R, 0), make_int2(0, GRID_R), pts[i]);
+        loc[i] = o > 0 ? t0 * 4 + 3 : (o < 0 ? (t0 + 1) * 4 + 3 : t0 * 4 + 0);
+        pend[i] = (int)i;
+    }
+}
+
+// constraint vertex ids: instance-local -> global; ids outside the instance
+// become -1 (rejected by the constraint checks)
+__global__ __launch_bounds__(kBlock) void k_batch_cons(const int2* in, const long long* coff, const long long* poff, int K,
+                                                       long long NC, int2* out)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NC; i += (long long)gridDim.x * blockDim.x) {
+        int lo = 0, hi = K;
+        while (lo + 1 < hi) {
+            int mid = (lo + hi) >> 1;
+            if (coff[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        long long base = poff[lo], n = poff[lo + 1] - poff[lo];
+        int2 e = in[i];
+        e.x = (e.x >= 0 && e.x < n) ? (int)(base + e.x) : -1;
+        e.y = (e.y >= 0 && e.y < n) ? (int)(base + e.y) : -1;
+        out[i] = e;
+    }
+}
+
+// scatter the global mesh back into per-instance lists with local vertex ids
+__global__ __launch_bounds__(kBlock) void k_batch_out(const int3* tri, int T, const int* vinst, const long long* poff,
+                                                      const long long* toff, long long NP, int* cursor, int3* out,
+                                                      int* err)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        int3 t = tri[i];
+        int inst = t.x < NP ? vinst[t.x] : (int)((t.x - NP) >> 2);
+        long long base = poff[inst], n = poff[inst + 1] - poff[inst];
+        auto local = [&](int v) -> int { return v < NP ? (int)(v - base) : (int)(n + ((v - NP) & 3)); };
+        long long pos = toff[inst] + atomicAdd(&cursor[inst], 1);
+        if (pos >= toff[inst + 1]) {
+            atomicOr(err, MERR_INTERNAL_COUNT);
+            continue;
+        }
+        out[pos] = make_int3(local(t.x), local(t.y), local(t.z));
+    }
+}
+
+void batch_vinst(Ctx& c, const long long* off, int K, long long NP, int* vinst)
+{
+    if (NP <= 0) return;
+    k_batch_vinst<<<grid_for(NP), kBlock, 0, c.st>>>(off, K, NP, vinst);
+    GCDT_LAUNCH_CHECK();
+}
+
+void batch_init(Ctx& c, int2* pts, int3* tri, int3* adj, int K, long long NP)
+{
+    k_batch_init<<<grid_for(K), kBlock, 0, c.st>>>(pts, tri, adj, K, NP);
+    GCDT_LAUNCH_CHECK();
+}
+
+void batch_locate(Ctx& c, const int2* pts, const int* vinst, long long NP, int* loc, int* pend)
+{
+    if (NP <= 0) return;
+    k_batch_locate<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, loc, pend);
+    GCDT_LAUNCH_CHECK();
+}
+
+void batch_cons(Ctx& c, const int2* in, const long long* coff, const long long* poff, int K, long long NC, int2* out)
+{
+    if (NC <= 0) return;
+    k_batch_cons<<<grid_for(NC), kBlock, 0, c.st>>>(in, coff, poff, K, NC, out);
+    GCDT_LAUNCH_CHECK();
+}
+
+void batch_out(Ctx& c, const int3* tri, int T, const int* vinst, const long long* poff, const long long* toff,
+               long long NP, int K, int3* out)
+{
+    DBuf<int> cursor(K);
+    fill_i32(c, cursor.p, 0, K);
+    k_batch_out<<<grid_for(T), kBlock, 0, c.st>>>(tri, T, vinst, poff, toff, NP, cursor.p, out, c.dcnt.p + C_ERR);
+    GCDT_LAUNCH_CHECK();
+    c.check_err();
 }
 
 // ================================================================== flags -> list

@@ -1,11 +1,24 @@
 // Host-side interface of the mesh phases (internal to the library).
 #pragma once
 #include <utility>
+#include <vector>
 
 #include "../../include/MemCache.h"
 #include "mesh_bodies.hpp"
 
 namespace gcdt {
+
+// Pool blocks released while a solve on a caller's stream may still have
+// kernels queued are parked here (per thread, while that solve's Ctx lives)
+// and returned to the pool once the stream has drained: a solve on another
+// stream could otherwise be handed a block that queued kernels still use.
+// Solves on the legacy null stream release at once -- that stream orders
+// every later use of the block.
+struct PendingFree {
+    void* p;
+    size_t bytes;
+};
+extern thread_local std::vector<PendingFree>* g_defer;
 
 // Device buffer from the MemCache pool, released on destruction.
 template <class T>
@@ -27,7 +40,13 @@ struct DBuf {
     }
     void release()
     {
-        if (p) getMemCacheRef().releaseAbtrtMem(p, sizeof(T) * (n ? n : 1));
+        if (p) {
+            size_t bytes = sizeof(T) * (n ? n : 1);
+            if (g_defer)
+                g_defer->push_back({p, bytes});
+            else
+                getMemCacheRef().releaseAbtrtMem(p, bytes);
+        }
         p = nullptr;
         n = 0;
     }
@@ -81,6 +100,15 @@ enum : int {
     C_NUM = 32
 };
 
+// Process-wide defaults of the round tunables (environment variables
+// GCDT_TAIL_THRESHOLD, GCDT_PIVOT, GCDT_SINGLE_TESTER override them at start).
+struct Tunables {
+    int tail_threshold = 4096;
+    int pivot = 0;
+    int single_tester = 1;
+    static Tunables& get();
+};
+
 struct Ctx {
     hipStream_t st = nullptr;
     DBuf<int> dcnt;
@@ -117,6 +145,9 @@ struct Ctx {
     const int* wait_snapshot(int slot);  // waits for that copy
     void zero_counters();
     void check_err();  // throws on a device error flag
+    // deferred pool releases of a solve on a non-null stream
+    std::vector<PendingFree> deferred;
+    std::vector<PendingFree>* prev_defer = nullptr;
 };
 
 // Live per-kernel timing (enabled through the C ABI): launches of the
@@ -160,6 +191,14 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
 void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv);
 void relabel_pairs(Ctx& c, const int2* in, int C, const int* inv, int n, int2* out);
 void unmap_tris(Ctx& c, int3* tri, int T, const int* perm, int n);
+// batches: instance map, per-instance corners/seed triangles/location, local
+// constraint ids -> global, and the per-instance output scatter
+void batch_vinst(Ctx& c, const long long* off, int K, long long NP, int* vinst);
+void batch_init(Ctx& c, int2* pts, int3* tri, int3* adj, int K, long long NP);
+void batch_locate(Ctx& c, const int2* pts, const int* vinst, long long NP, int* loc, int* pend);
+void batch_cons(Ctx& c, const int2* in, const long long* coff, const long long* poff, int K, long long NC, int2* out);
+void batch_out(Ctx& c, const int3* tri, int T, const int* vinst, const long long* poff, const long long* toff,
+               long long NP, int K, int3* out);
 // indices i with flags[i] != 0 (flags are 0/1), ascending; returns the count
 int flags_to_list(Ctx& c, const int* flags, int n, int* list);
 // leaves of a DAG -> compact arrays (adjacency remapped); returns count
@@ -189,11 +228,12 @@ struct ConsResult {
     long long pieces = 0;
     long long rewritten = 0;
 };
-ConsResult insert_constraints(Ctx& c, const Mesh& m, int T, int nv, const int2* cons, int C, int* touched,
+// nmax: constraint endpoints must be vertex ids in [0, nmax)
+ConsResult insert_constraints(Ctx& c, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C, int* touched,
                               int* ntouched);
 // mark every edge chain of the mesh that runs exactly along a constraint
 // (constraints split at collinear vertices) as constrained; returns pieces
-long long mark_constraint_chains(Ctx& c, const Mesh& m, int T, int nv, const int2* cons, int C);
+long long mark_constraint_chains(Ctx& c, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C);
 // vertex ids n..n+3 <- the square corners (insertCornerPoints)
 void write_corners(Ctx& c, int2* pts, int n);
 

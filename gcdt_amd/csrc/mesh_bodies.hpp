@@ -55,6 +55,7 @@ enum : int {
     MERR_CAVITY = 64,      // cavity retriangulation inconsistency
     MERR_SCRATCH = 128,    // cavity scratch exhausted
     MERR_INVALID = 256,    // input outside the contract (coordinates, ids)
+    MERR_INTERNAL_COUNT = 512,  // per-instance triangle count mismatch
 };
 
 struct Mesh {

@@ -41,6 +41,18 @@ class Stats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double), ("ms_insert", ctypes.c_double), ("ms_flip", ctypes.c_double),
+        ("ms_constraints", ctypes.c_double), ("insert_rounds", ctypes.c_int), ("flip_rounds", ctypes.c_int),
+        ("constraint_rounds", ctypes.c_int), ("num_instances", ctypes.c_int), ("num_points", ctypes.c_int64),
+        ("num_triangles", ctypes.c_int64), ("flips", ctypes.c_int64), ("segments", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 vp = ctypes.c_void_p
 pvp = ctypes.POINTER(ctypes.c_void_p)
 pint = ctypes.POINTER(ctypes.c_int)
@@ -63,6 +75,11 @@ SIGNATURES = {
     "gcdt_memcpy_d2d": ([vp, vp, sz], i32),
     "gcdt_memset": ([vp, i32, sz], i32),
     "gcdt_scratch_bytes": ([ctypes.c_int64], sz),
+    "gcdt_stream_create": ([pvp], i32),
+    "gcdt_stream_destroy": ([vp], i32),
+    "gcdt_stream_synchronize": ([vp], i32),
+    "gcdt_memcpy_h2d_async": ([vp, vp, sz, vp], i32),
+    "gcdt_memcpy_d2h_async": ([vp, vp, sz, vp], i32),
     "gcdt_prof_enable": ([i32], i32),
     "gcdt_prof_reset": ([], i32),
     "gcdt_prof_read": ([ctypes.c_char_p, pdbl, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)], i32),
@@ -79,6 +96,10 @@ SIGNATURES = {
     "gcdt_delaunay_new": ([vp, sz, pvp, pvp, vp, i32, i32, i32, pdbl], i32),
     "gcdt_add_constraints": ([vp, sz, vp, vp, vp, vp, vp, vp, vp, pvp, i32, i32, i32, pint, pdbl], i32),
     "gcdt_solve": ([vp, i32, vp, i32, vp, vp, pint, ctypes.POINTER(Stats), vp], i32),
+    "gcdt_solve_batch": ([vp, ctypes.POINTER(ctypes.c_int64), i32, vp, ctypes.POINTER(ctypes.c_int64), vp,
+                          ctypes.POINTER(BatchStats), vp], i32),
+    "gcdt_set_tunable": ([ctypes.c_char_p, i32], i32),
+    "gcdt_get_tunable": ([ctypes.c_char_p, pint], i32),
 }
 
 _lib = None

@@ -65,6 +65,13 @@ int gcdt_memcpy_d2h(void* host_dst, const void* src, size_t bytes);
 int gcdt_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int gcdt_memset(void* dst, int value, size_t bytes);
 
+/* Non-blocking HIP streams (default priority) for concurrent solves. */
+int gcdt_stream_create(void** stream);
+int gcdt_stream_destroy(void* stream);
+int gcdt_stream_synchronize(void* stream);
+int gcdt_memcpy_h2d_async(void* dst, const void* host_src, size_t bytes, void* stream);
+int gcdt_memcpy_d2h_async(void* host_dst, const void* src, size_t bytes, void* stream);
+
 /* scratch bytes the (tmp, tmpBytes) argument must offer for n elements */
 size_t gcdt_scratch_bytes(int64_t n);
 
@@ -178,6 +185,41 @@ typedef struct {
  * receives the adjacency.  stream may be NULL (default stream). */
 int gcdt_solve(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcdt_int3* tris_out,
                gcdt_int3* adj_out, int* num_tris, gcdt_stats* stats, void* stream);
+
+/* ------------------------------------------------------------ batches */
+
+typedef struct {
+    double ms_total;
+    double ms_insert;      /* location + insertion rounds of all instances */
+    double ms_flip;
+    double ms_constraints;
+    int insert_rounds;
+    int flip_rounds;
+    int constraint_rounds;
+    int num_instances;
+    int64_t num_points;
+    int64_t num_triangles;
+    int64_t flips;
+    int64_t segments;
+} gcdt_batch_stats;
+
+/* K independent CDTs in one set of launches.  Instance i owns points
+ * [point_offsets[i], point_offsets[i+1]) of `points` (device, 2^30 grid,
+ * distinct within the instance) and constraints [cons_offsets[i],
+ * cons_offsets[i+1]) of `cons` (device, vertex ids LOCAL to the instance).
+ * The offsets arrays are HOST arrays of K+1 entries (cons_offsets may be
+ * NULL when there are no constraints).  Instance i's 2*N_i+2 triangles go to
+ * tris_out[sum_{j<i} (2*N_j+2) ...] with local vertex ids (corners
+ * N_i..N_i+3).  Every round of every phase runs for all instances at once,
+ * so a batch costs about as many rounds as its largest instance. */
+int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int K, const gcdt_int2* cons,
+                     const int64_t* cons_offsets, gcdt_int3* tris_out, gcdt_batch_stats* stats, void* stream);
+
+/* process-wide tunables read by every later solve: "tail_threshold"
+ * (list length below which rounds finish in one persistent workgroup),
+ * "pivot" (0 random, 1 central-first), "single_tester" (0/1). */
+int gcdt_set_tunable(const char* name, int value);
+int gcdt_get_tunable(const char* name, int* value);
 
 #ifdef __cplusplus
 }
