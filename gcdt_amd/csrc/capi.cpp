@@ -14,11 +14,14 @@ using namespace gcdt;
 
 static thread_local std::string g_last_error;
 
-static int fail(int status, const std::string& msg)
+namespace gcdt {
+// records the calling thread's error message and returns the status
+int fail(int status, const std::string& msg)
 {
     g_last_error = msg;
     return status;
 }
+}  // namespace gcdt
 
 #define API_BEGIN try {
 #define API_END                                             \

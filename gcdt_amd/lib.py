@@ -80,6 +80,11 @@ SIGNATURES = {
     "gcdt_stream_synchronize": ([vp], i32),
     "gcdt_memcpy_h2d_async": ([vp, vp, sz, vp], i32),
     "gcdt_memcpy_d2h_async": ([vp, vp, sz, vp], i32),
+    "gcdt_read_file": ([ctypes.c_char_p, i32, ctypes.POINTER(ctypes.POINTER(ctypes.c_double)), pint, pvp, pint], i32),
+    "gcdt_free_host": ([vp], None),
+    "gcdt_trans_points": ([vp, i32, ctypes.c_longlong, vp, vp], i32),
+    "gcdt_canonicalize_host": ([vp, ctypes.c_int64], i32),
+    "gcdt_write_mesh": ([ctypes.c_char_p, vp, i32, vp, ctypes.c_int64], i32),
     "gcdt_prof_enable": ([i32], i32),
     "gcdt_prof_reset": ([], i32),
     "gcdt_prof_read": ([ctypes.c_char_p, pdbl, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)], i32),

@@ -221,6 +221,42 @@ int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int 
 int gcdt_set_tunable(const char* name, int value);
 int gcdt_get_tunable(const char* name, int* value);
 
+/* ------------------------------------------------------------ I/O formats */
+
+/* kernel.cu:5159-5291 readFile.  Text file: a point count, then one line per
+ * point "idx x y" (idx in [0,count), x and y doubles), then an optional
+ * constraint count and one line per constraint "k a b" (a, b point idx).
+ * Points with identical coordinates are merged (the first keeps its slot,
+ * later idx map onto it); the rest are scaled about the bounding-box centre
+ * by the longer side into [0.005, 0.995]^2 as ((v-c)/l)*0.99+0.5.  With
+ * shuffle != 0 point i moves to slot random[i], random = std::shuffle of
+ * 0..n-1 under std::mt19937(19999) (the permutation is the host C++ library's).
+ * Constraints are mapped through the merge (and the shuffle); self-loops and
+ * repeats in either orientation are dropped.  Outputs are HOST arrays
+ * (2*numPoints doubles, numCons pairs) released with gcdt_free_host.
+ * Deviations: the bounding box starts from the first point (the reference
+ * starts from +-1e6, which is wrong when every coordinate lies beyond 1e6), a
+ * zero extent scales by 1, and malformed files fail with GCDT_ERR_INVALID. */
+int gcdt_read_file(const char* path, int shuffle, double** points, int* numPoints, gcdt_int2** cons,
+                   int* numCons);
+void gcdt_free_host(void* ptr);
+
+/* kernel.cu:171-188 transPoints: device double[2n] -> device int2[n] on the
+ * grid of resolution res (2^30 in the reference): trunc(v*res) clamped to
+ * [0, res-1]. */
+int gcdt_trans_points(const double* fpoints, int n, long long res, gcdt_int2* points, void* stream);
+
+/* Canonical triangle list, in place on HOST memory: each triangle rotated so
+ * that its smallest vertex id comes first (orientation kept), then sorted
+ * lexicographically.  Two triangulations of the same points are equal iff
+ * their canonical lists are. */
+int gcdt_canonicalize_host(gcdt_int3* tris, int64_t numTriangles);
+
+/* Writes HOST points and triangles as text: the point count, "i x y" lines,
+ * the triangle count, "i a b c" lines (the readFile layout, extended). */
+int gcdt_write_mesh(const char* path, const gcdt_int2* points, int numPoints, const gcdt_int3* tris,
+                    int64_t numTriangles);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,9 +10,10 @@
 //   include/rawMesh.h:7   rawMesh                     rawMesh
 //   include/boostmap.h:3  getBitmap                   getBitmap
 //   include/addConstraints.h:5  addConstraints (14 args)   addConstraints
-//   include/addConstraints.h:6  addConstraints (10 args, a stub returning 0
-//                               in addConstraints.cu:1837-1840) -> recovers
-//                               the constraints in place of the stub
+//   include/addConstraints.h:6  addConstraints (10 args)   addConstraints
+//                               (the reference's body is a stub that returns 0
+//                               and changes nothing, addConstraints.cu:1837-1840;
+//                               kept as the same no-op so callers link)
 //   include/fastDelaunay.h:5  tryFastDelaunay         tryFastDelaunay
 //   kernel.cu:5038 delaunaySample, :4697 delaunayNew, :5293 filterTriangles,
 //   :833 convertSample (header-less in the reference)
@@ -100,6 +101,9 @@ inline double addConstraints(void* tmp, size_t bytes, Point2d* points, int3* tri
     newTris = reinterpret_cast<int3*>(nt);
     return ms;
 }
+
+// include/addConstraints.h:6 -- no-op, as in the reference (addConstraints.cu:1837-1840)
+inline double addConstraints(void*, size_t, Point2d*, int3*, int3*, int*, int2*, int, int, int) { return 0.0; }
 
 inline double tryFastDelaunay(void* tmp, size_t bytes, Point2d* points, int3* tris, int3* adjTris, int numPoints,
                               int numTriangles)
