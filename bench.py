@@ -44,6 +44,24 @@ BYTES_PER_ITEM = {
 }
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cdt_10M_clustered_hbm_traffic.md")
+
+
+def _traffic_per_item():
+    """Measured HBM bytes per list item (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    scaled by a 1 GiB copy) from the committed profile summary."""
+    out = {}
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_item "):
+                    _, k, v = line.split()
+                    out[k] = float(v)
+    except OSError:
+        pass
+    return out
+
+
 def _prof(lib, name):
     import ctypes
     ms = ctypes.c_double(0)
@@ -66,6 +84,9 @@ def main(argv=None):
     ap.add_argument("--batch-count", type=int, default=4096, help="instances in the batched workload (0 = skip)")
     ap.add_argument("--per-instance", type=int, default=256,
                     help="instances timed with one plain solve each, for the batched comparison")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the solves, copy MIB MiB device-to-device 3 times (a plain copy kernel of "
+                         "known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
     ap.add_argument("--serve-count", type=int, default=128, help="requests in the serving run (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving run")
     ap.add_argument("--serve-workers", type=int, default=3, help="worker streams of the serving run")
@@ -88,6 +109,13 @@ def main(argv=None):
     n, c = len(g), len(cons)
     print(f"# workload: {n} points, {c} segments (generated in {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
 
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        with DeviceBuffer(nb) as a, DeviceBuffer(nb) as b:
+            check(L.gcdt_memset(a.ptr, 1, nb), "memset")
+            for _ in range(3):
+                check(L.gcdt_memcpy_d2d(b.ptr, a.ptr, nb), "copy")
+            check(L.gcdt_synchronize(), "sync")
     dp = DeviceBuffer.from_array(g)
     dc = DeviceBuffer.from_array(cons)
     dt = DeviceBuffer(12 * (2 * n + 2))
@@ -131,8 +159,12 @@ def main(argv=None):
         avg_ms = ms / max(launches, 1)
         bytes_per_launch = BYTES_PER_ITEM[dom] * items / max(launches, 1)
         achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
+        tpi = _traffic_per_item().get(dom)
+        traffic = round(tpi * items / max(launches, 1)) if tpi else None
         roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_GBs": round(traffic / (avg_ms / 1000.0) / 1e9, 1) if traffic and avg_ms > 0 else None,
+                "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                 "avg_launch_us": round(avg_ms * 1000.0, 3), "launches_per_step": launches // args.steps,
                 "kernel_ms_per_step": round(ms / args.steps, 3),
                 "bytes_per_item": BYTES_PER_ITEM[dom], "items_per_launch": items // max(launches, 1)}
