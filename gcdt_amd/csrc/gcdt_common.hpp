@@ -108,7 +108,9 @@ __device__ inline int block_alloc(int k, int* counter)
 __device__ __host__ inline int nx3(int k) { return k == 2 ? 0 : k + 1; }
 __device__ __host__ inline int pv3(int k) { return k == 0 ? 2 : k - 1; }
 
-__device__ __host__ inline int get3(const int3& t, int k) { return k == 0 ? t.x : (k == 1 ? t.y : t.z); }
+// by value: with a reference, the selects fold into a computed field offset
+// and the caller's local int3 lands on the stack
+__device__ __host__ inline int get3(int3 t, int k) { return k == 0 ? t.x : (k == 1 ? t.y : t.z); }
 __device__ __host__ inline void set3(int3& t, int k, int v)
 {
     if (k == 0) t.x = v;

@@ -951,18 +951,52 @@ __global__ void k_flip_begin(int* cnt, int move)
     cnt[C_NFLIP] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_flip_test(Flip f, const int* list, const int* cnt, unsigned epoch,
-                                                     int single)
+// Flip engines: the array-per-field layout (short lists on a mesh whose
+// arrays stay in place) and the packed 64-byte records (whole-mesh rounds).
+// Both keep the same claims protocol; apply() sets mark = epoch + 1 on the
+// two triangles of a flip.
+struct EngSoA {
+    Flip f;
+    __device__ void test(int t, unsigned ep, bool all, bool single) const { flip_test(f, t, ep, all, single); }
+    template <bool COH>
+    __device__ int apply(int t, unsigned ep, int* nn) const
+    {
+        int res = flip_apply<COH>(f, t, ep, nn);
+        if (res == 2) {
+            f.mark[t] = (int)ep + 1;
+            f.mark[*nn] = (int)ep + 1;
+        }
+        return res;
+    }
+    __device__ void fix(int t, unsigned ep) const { flip_fix(f, t, ep); }
+    __device__ void set_mark(int t, int v) const { f.mark[t] = v; }
+};
+
+struct EngPacked {
+    FlipP f;
+    __device__ void test(int t, unsigned ep, bool all, bool single) const { flip_test_p(f, t, ep, all, single); }
+    template <bool COH>
+    __device__ int apply(int t, unsigned ep, int* nn) const
+    {
+        return flip_apply_p<COH>(f, t, ep, nn);
+    }
+    __device__ void fix(int t, unsigned ep) const { flip_fix_p(f, t, ep); }
+    __device__ void set_mark(int t, int v) const { f.r[t].mark = v; }
+};
+
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_flip_test(E e, const int* list, const int* cnt, unsigned epoch, int single)
 {
     int n = *cnt;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        flip_test(f, list ? list[i] : (int)i, epoch, list == nullptr, single != 0);
+        e.test(list ? list[i] : (int)i, epoch, list == nullptr, single != 0);
 }
 
 constexpr int FA_ITEMS = 4;  // list items per thread per block-wide append
 
-__global__ __launch_bounds__(kBlock) void k_flip_apply(Flip f, const int* list, const int* cnt, unsigned epoch,
-                                                       int* list2, int* cnt2, int* flipped, int* nflip)
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_flip_apply(E e, const int* list, const int* cnt, unsigned epoch, int* list2,
+                                                       int* cnt2, int* flipped, int* nflip)
 {
     int n = *cnt;
     TILE_LOOP(base, n, kBlock * FA_ITEMS)
@@ -977,7 +1011,8 @@ __global__ __launch_bounds__(kBlock) void k_flip_apply(Flip f, const int* list, 
             if (i < n) {
                 int t = list ? list[i] : (int)i;
                 int nn = -1;
-                int res = flip_apply<false>(f, t, epoch, &nn);
+                int res = e.template apply<false>(t, epoch, &nn);
+                if (res == 1) e.set_mark(t, (int)epoch + 1);
                 if (res == 1 || res == 2) qa[j] = t;
                 if (res == 2) qb[j] = nn;
             }
@@ -988,13 +1023,9 @@ __global__ __launch_bounds__(kBlock) void k_flip_apply(Flip f, const int* list, 
         int fb = block_alloc<kBlock>(nf, nflip);
 #pragma unroll
         for (int j = 0; j < FA_ITEMS; j++) {
-            if (qa[j] >= 0) {
-                list2[b++] = qa[j];
-                f.mark[qa[j]] = (int)epoch + 1;
-            }
+            if (qa[j] >= 0) list2[b++] = qa[j];
             if (qb[j] >= 0) {
                 list2[b++] = qb[j];
-                f.mark[qb[j]] = (int)epoch + 1;
                 flipped[fb++] = qa[j];
                 flipped[fb++] = qb[j];
             }
@@ -1002,15 +1033,17 @@ __global__ __launch_bounds__(kBlock) void k_flip_apply(Flip f, const int* list, 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_flip_fix(Flip f, const int* list, const int* cnt, unsigned epoch)
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_flip_fix(E e, const int* list, const int* cnt, unsigned epoch)
 {
     int n = *cnt;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        flip_fix(f, list[i], epoch);
+        e.fix(list[i], epoch);
 }
 
 // Persistent single-workgroup tail of the flip rounds.
-__global__ __launch_bounds__(TAIL_BS) void k_flip_tail(Flip f, int* A, int* B, int* lf, int* cnt, unsigned epoch0,
+template <class E>
+__global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int* lf, int* cnt, unsigned epoch0,
                                                        int max_rounds)
 {
     __shared__ int s_n2, s_nf;
@@ -1018,7 +1051,7 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(Flip f, int* A, int* B, i
     unsigned ep = epoch0;
     int r = 0, err = 0, flips = 0;
     while (n > 0 && r < max_rounds) {
-        for (int i = threadIdx.x; i < n; i += TAIL_BS) flip_test(f, A[i], ep, false, false);
+        for (int i = threadIdx.x; i < n; i += TAIL_BS) e.test(A[i], ep, false, false);
         if (threadIdx.x == 0) {
             s_n2 = 0;
             s_nf = 0;
@@ -1026,16 +1059,14 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(Flip f, int* A, int* B, i
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += TAIL_BS) {
             int t = A[i], nn = -1;
-            int res = flip_apply(f, t, ep, &nn);
+            int res = e.template apply<true>(t, ep, &nn);
             if (res == 1) {
                 B[atomicAdd(&s_n2, 1)] = t;
-                f.mark[t] = (int)ep + 1;
+                e.set_mark(t, (int)ep + 1);
             } else if (res == 2) {
                 int b = atomicAdd(&s_n2, 2);
                 B[b] = t;
                 B[b + 1] = nn;
-                f.mark[t] = (int)ep + 1;
-                f.mark[nn] = (int)ep + 1;
                 int q = atomicAdd(&s_nf, 2);
                 lf[q] = t;
                 lf[q + 1] = nn;
@@ -1043,7 +1074,7 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(Flip f, int* A, int* B, i
         }
         __syncthreads();
         int nf = s_nf, n2 = s_n2;
-        for (int i = threadIdx.x; i < nf; i += TAIL_BS) flip_fix(f, lf[i], ep);
+        for (int i = threadIdx.x; i < nf; i += TAIL_BS) e.fix(lf[i], ep);
         __syncthreads();
         if (n2 > 0 && nf == 0) {
             err = 1;
@@ -1065,10 +1096,11 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(Flip f, int* A, int* B, i
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_mark_list(const int* list, int n, int* mark, int v)
+template <class E>
+__global__ __launch_bounds__(kBlock) void k_mark_list(E e, const int* list, int n, int v)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        mark[list[i]] = v;
+        e.set_mark(list ? list[i] : (int)i, v);
 }
 
 __global__ __launch_bounds__(kBlock) void k_iota(int* p, int n)
@@ -1077,29 +1109,54 @@ __global__ __launch_bounds__(kBlock) void k_iota(int* p, int n)
         p[i] = (int)i;
 }
 
-long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds)
+__global__ __launch_bounds__(kBlock) void k_pack_faces(const int3* __restrict__ tri, const int3* __restrict__ adj,
+                                                       const int* __restrict__ cflag, const int2* __restrict__ pts,
+                                                       int T, FRec* __restrict__ r)
 {
-    DBuf<unsigned long long> claim(T);
-    DBuf<int> stamp(T), partner(T), cand(T), mark(T), la(T), lb(T), lf(T);
-    fill_u64(c, claim.p, 0, T);
-    fill_i32(c, stamp.p, 0, T);
-    fill_i32(c, mark.p, 0, T);
-    Flip f;
-    f.m = m;
-    f.claim = claim.p;
-    f.stamp = stamp.p;
-    f.partner = partner.p;
-    f.cand = cand.p;
-    f.mark = mark.p;
-    f.err = c.dcnt.p + C_ERR;
-    // round 1 reads `init` (or the implicit list of all T triangles, counted
-    // in C_CONST); later rounds alternate between la and lb, counted in C_NACT
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (long long)gridDim.x * blockDim.x) {
+        FRec x;
+        x.v = tri[t];
+        x.a = adj[t];
+        x.p[0] = pts[x.v.x];
+        x.p[1] = pts[x.v.y];
+        x.p[2] = pts[x.v.z];
+        x.meta = (cflag ? cflag[t] & 7 : 0) | FREC_NOCAND;
+        x.mark = 0;
+        x.stamp = 0;
+        x.partner = -1;
+        r[t] = x;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_unpack_faces(const FRec* __restrict__ r, int T, int3* __restrict__ tri,
+                                                         int3* __restrict__ adj, int* __restrict__ cflag)
+{
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (long long)gridDim.x * blockDim.x) {
+        const FRec x = r[t];
+        tri[t] = x.v;
+        adj[t] = x.a;
+        if (cflag) cflag[t] = x.meta & 7;
+    }
+}
+
+namespace {
+
+// Rounds of test / apply / fix over a triangle list until no edge needs a
+// flip.  Round 1 reads `init` (or the implicit list of all T triangles,
+// counted in C_CONST); later rounds alternate between two device lists whose
+// lengths stay on the device (C_NACT); the host checks round r-1 while round
+// r is queued, and lists at or below the tail threshold finish in one
+// persistent workgroup.
+template <class E>
+long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int* rounds)
+{
+    DBuf<int> la(T), lb(T), lf(T);
     const int* act = init;
     int* nxt = la.p;
     int* other = lb.p;
     if (init) {
         c.set(C_NACT, ninit);
-        k_mark_list<<<grid_for(ninit), kBlock, 0, c.st>>>(init, ninit, mark.p, (int)c.epoch + 1);
+        k_mark_list<E><<<grid_for(ninit), kBlock, 0, c.st>>>(e, init, ninit, (int)c.epoch + 1);
         GCDT_LAUNCH_CHECK();
     } else {
         c.set(C_CONST, T);
@@ -1123,18 +1180,18 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
         int g = grid_for(est);
         {
             ProfScope ps("flip_test", c.st, est);
-            k_flip_test<<<g, kBlock, 0, c.st>>>(f, act, c.dcnt.p + cnt_idx, ep, c.single_tester);
+            k_flip_test<E><<<g, kBlock, 0, c.st>>>(e, act, c.dcnt.p + cnt_idx, ep, c.single_tester);
             GCDT_LAUNCH_CHECK();
         }
         {
             ProfScope ps("flip_apply", c.st, est);
-            k_flip_apply<<<grid_for((est + FA_ITEMS - 1) / FA_ITEMS), kBlock, 0, c.st>>>(
-                f, act, c.dcnt.p + cnt_idx, ep, nxt, c.dcnt.p + C_NACT2, lf.p, c.dcnt.p + C_NFLIP);
+            k_flip_apply<E><<<grid_for((est + FA_ITEMS - 1) / FA_ITEMS), kBlock, 0, c.st>>>(
+                e, act, c.dcnt.p + cnt_idx, ep, nxt, c.dcnt.p + C_NACT2, lf.p, c.dcnt.p + C_NFLIP);
             GCDT_LAUNCH_CHECK();
         }
         {
             ProfScope ps("flip_fix", c.st, 0);
-            k_flip_fix<<<grid_for(est), kBlock, 0, c.st>>>(f, lf.p, c.dcnt.p + C_NFLIP, ep);
+            k_flip_fix<E><<<grid_for(est), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, ep);
             GCDT_LAUNCH_CHECK();
         }
         c.snapshot(r & 1);
@@ -1159,7 +1216,8 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
         if (!act) {  // small mesh: materialise the implicit list
             k_iota<<<grid_for(est), kBlock, 0, c.st>>>(lb.p, est);
             GCDT_LAUNCH_CHECK();
-            fill_i32(c, mark.p, (int)ep0, est);
+            k_mark_list<E><<<grid_for(est), kBlock, 0, c.st>>>(e, nullptr, est, (int)ep0);
+            GCDT_LAUNCH_CHECK();
             act = lb.p;
             nxt = la.p;
             c.set(C_NACT, est);
@@ -1169,7 +1227,7 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
         }
         {
             ProfScope ps("flip_tail", c.st, est);
-            k_flip_tail<<<1, TAIL_BS, 0, c.st>>>(f, (int*)act, nxt, lf.p, c.dcnt.p, ep0, 1 << 20);
+            k_flip_tail<E><<<1, TAIL_BS, 0, c.st>>>(e, (int*)act, nxt, lf.p, c.dcnt.p, ep0, 1 << 20);
             GCDT_LAUNCH_CHECK();
         }
         c.read(0, C_NUM);
@@ -1182,6 +1240,48 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
     }
     if (rounds) *rounds = r;
     return flips;
+}
+
+}  // namespace
+
+long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds)
+{
+    if (T <= 0) {
+        if (rounds) *rounds = 0;
+        return 0;
+    }
+    DBuf<unsigned long long> claim(T);
+    fill_u64(c, claim.p, 0, T);
+    if (!init) {
+        // whole mesh: packed records for the rounds, written back at the end
+        DBuf<FRec> rec(T);
+        {
+            ProfScope ps("flip_pack", c.st, T);
+            k_pack_faces<<<grid_for(T), kBlock, 0, c.st>>>(m.tri, m.adj, m.cflag, m.pts, T, rec.p);
+            GCDT_LAUNCH_CHECK();
+        }
+        EngPacked e;
+        e.f = FlipP{rec.p, claim.p, c.dcnt.p + C_ERR};
+        long long flips = flip_driver(c, e, T, nullptr, 0, rounds);
+        {
+            ProfScope ps("flip_unpack", c.st, T);
+            k_unpack_faces<<<grid_for(T), kBlock, 0, c.st>>>(rec.p, T, m.tri, m.adj, m.cflag);
+            GCDT_LAUNCH_CHECK();
+        }
+        return flips;
+    }
+    DBuf<int> stamp(T), partner(T), cand(T), mark(T);
+    fill_i32(c, stamp.p, 0, T);
+    fill_i32(c, mark.p, 0, T);
+    EngSoA e;
+    e.f.m = m;
+    e.f.claim = claim.p;
+    e.f.stamp = stamp.p;
+    e.f.partner = partner.p;
+    e.f.cand = cand.p;
+    e.f.mark = mark.p;
+    e.f.err = c.dcnt.p + C_ERR;
+    return flip_driver(c, e, T, init, ninit, rounds);
 }
 
 // ================================================================== batches of instances

@@ -403,4 +403,149 @@ __host__ __device__ inline void flip_fix(const Flip& f, int t, unsigned epoch)
     m.adj[t] = ta;
 }
 
+// ---------------------------------------------------------------- packed flip records
+
+// One 64-byte record per triangle for flip rounds over a whole mesh.  A test
+// or a flip reads a neighbour's vertices, neighbours, vertex coordinates and
+// flip bookkeeping from one record -- one cache-line visit -- where the
+// array-per-field layout (tri, adj, pts, cflag, cand, mark, stamp, partner)
+// gathers from up to eight places.
+struct FRec {
+    int3 v;       // vertices, counter-clockwise
+    int3 a;       // neighbour across the edge opposite v[k]; marked while a flip's outer edges settle
+    int2 p[3];    // coordinates of v[0..2]
+    int meta;     // bits 0-2: constrained edges (bit k = edge opposite v[k]); bits 3-4: proposed edge, 3 = none
+    int mark;     // epoch in which the triangle is in the active list
+    int stamp;    // epoch of its last flip
+    int partner;  // the other triangle of that flip
+};
+static_assert(sizeof(FRec) == 64, "flip record is one 64-byte line (arrays of records start 256-byte aligned)");
+
+constexpr int FREC_NOCAND = 3 << 3;
+
+// Whole-record load and store as four 16-byte accesses (a struct copy of an
+// over-aligned local is otherwise staged through the stack).
+__host__ __device__ inline FRec rec_load(const FRec* p)
+{
+    const int4* q = reinterpret_cast<const int4*>(p);
+    int4 a = q[0], b = q[1], c = q[2], d = q[3];
+    FRec r;
+    r.v = make_int3(a.x, a.y, a.z);
+    r.a = make_int3(a.w, b.x, b.y);
+    r.p[0] = make_int2(b.z, b.w);
+    r.p[1] = make_int2(c.x, c.y);
+    r.p[2] = make_int2(c.z, c.w);
+    r.meta = d.x;
+    r.mark = d.y;
+    r.stamp = d.z;
+    r.partner = d.w;
+    return r;
+}
+
+__host__ __device__ inline void rec_store(FRec* p, const int3& v, const int3& a, const int2& p0, const int2& p1,
+                                          const int2& p2, int meta, int mark, int stamp, int partner)
+{
+    int4* q = reinterpret_cast<int4*>(p);
+    q[0] = make_int4(v.x, v.y, v.z, a.x);
+    q[1] = make_int4(a.y, a.z, p0.x, p0.y);
+    q[2] = make_int4(p1.x, p1.y, p2.x, p2.y);
+    q[3] = make_int4(meta, mark, stamp, partner);
+}
+
+// coordinate k of a record by selects over values (a runtime index into the
+// local copy, or selects through a reference, would put it on the stack)
+__host__ __device__ inline int2 sel_pt(int2 p0, int2 p1, int2 p2, int k)
+{
+    return make_int2(k == 0 ? p0.x : (k == 1 ? p1.x : p2.x), k == 0 ? p0.y : (k == 1 ? p1.y : p2.y));
+}
+#define rec_pt(r, k) sel_pt((r).p[0], (r).p[1], (r).p[2], (k))
+
+struct FlipP {
+    FRec* r;
+    unsigned long long* claim;
+    int* err;
+};
+
+// flip_test over records (same rule as flip_test).
+__host__ __device__ inline void flip_test_p(const FlipP& f, int t, unsigned epoch, bool all_active, bool single = true)
+{
+    const FRec r = rec_load(&f.r[t]);
+    const int cf = r.meta & 7;
+    int cand = 3;
+    for (int k = 0; k < 3; k++) {
+        if ((cf >> k) & 1) continue;
+        int n = get3(r.a, k);
+        if (n < 0) continue;
+        bool lower = single && n < t;
+        if (lower && all_active) continue;
+        const FRec rn = rec_load(&f.r[n]);
+        if (lower && rn.mark == (int)epoch) continue;
+        int j = slot_other(rn.v, get3(r.v, nx3(k)), get3(r.v, pv3(k)));
+        if (incircle(r.p[0], r.p[1], r.p[2], rec_pt(rn, j)) > 0) {
+            cand = k;
+            unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
+            at_max(&f.claim[t], key);
+            at_max(&f.claim[n], key);
+            break;
+        }
+    }
+    f.r[t].meta = cf | (cand << 3);
+}
+
+// flip_apply over records: same claims protocol and return codes.  A flip
+// rewrites both records whole, including mark = epoch + 1 (both are queued).
+template <bool COH = true>
+__host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoch, int* nout)
+{
+    const int k = (f.r[t].meta >> 3) & 3;
+    if (k == 3) return 0;
+    const FRec r = rec_load(&f.r[t]);
+    const int n = get3(r.a, k);
+    const unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
+    const unsigned long long ct = ld_claim<COH>(&f.claim[t]);
+    if (ct != key || ld_claim<COH>(&f.claim[n]) != key) {
+        if (ct != key) {
+            int t2 = (int)((ct & 0xffffffffu) / 3);
+            if ((ct >> 32) == epoch && ld_claim<COH>(&f.claim[t2]) == ct) return 3;
+        }
+        return 1;
+    }
+    const FRec rn = rec_load(&f.r[n]);
+    const int kn = nx3(k), kp = pv3(k);
+    const int p = get3(r.v, k), u = get3(r.v, kn), v = get3(r.v, kp);
+    const int j = slot_other(rn.v, u, v);
+    const int jn = nx3(j), jp = pv3(j);
+    const int o = get3(rn.v, j);
+    const int cft = r.meta & 7, cfn = rn.meta & 7;
+    const int2 pp = rec_pt(r, k), pu = rec_pt(r, kn), pv = rec_pt(r, kp), po = rec_pt(rn, j);
+    // t = (p,u,v), n = (o,v,u)  ->  t = (p,u,o), n = (p,o,v)
+    rec_store(&f.r[t], make_int3(p, u, o), make_int3(adj_mark(get3(rn.a, jn)), n, adj_mark(get3(r.a, kp))), pp, pu, po,
+              ((cfn >> jn) & 1) | (((cft >> kp) & 1) << 2) | FREC_NOCAND, (int)epoch + 1, (int)epoch, n);
+    rec_store(&f.r[n], make_int3(p, o, v), make_int3(adj_mark(get3(rn.a, jp)), adj_mark(get3(r.a, kn)), t), pp, po, pv,
+              ((cfn >> jp) & 1) | (((cft >> kn) & 1) << 1) | FREC_NOCAND, (int)epoch + 1, (int)epoch, t);
+    *nout = n;
+    return 2;
+}
+
+// flip_fix over records.
+__host__ __device__ inline void flip_fix_p(const FlipP& f, int t, unsigned epoch)
+{
+    const FRec r = rec_load(&f.r[t]);
+    int3 ta = r.a;
+    for (int k = 0; k < 3; k++) {
+        int av = get3(ta, k);
+        if (!adj_is_mark(av)) continue;
+        int nb = adj_unmark(av);
+        int u = get3(r.v, nx3(k)), v = get3(r.v, pv3(k));
+        const FRec rb = rec_load(&f.r[nb]);
+        if (rb.stamp == (int)epoch) {
+            set3(ta, k, has2(rb.v, u, v) ? nb : rb.partner);
+        } else {
+            set3(ta, k, nb);
+            (&f.r[nb].a.x)[slot_other(rb.v, u, v)] = t;
+        }
+    }
+    f.r[t].a = ta;
+}
+
 }  // namespace gcdt
