@@ -87,6 +87,8 @@ def main(argv=None):
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the solves, copy MIB MiB device-to-device 3 times (a plain copy kernel of "
                          "known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tunables (skip the autotuner)")
+    ap.add_argument("--no-degenerate", action="store_true", help="skip the 4096^2 lattice validity run")
     ap.add_argument("--serve-count", type=int, default=128, help="requests in the serving run (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving run")
     ap.add_argument("--serve-workers", type=int, default=3, help="worker streams of the serving run")
@@ -120,6 +122,15 @@ def main(argv=None):
     dc = DeviceBuffer.from_array(cons)
     dt = DeviceBuffer(12 * (2 * n + 2))
     da = DeviceBuffer(12 * (2 * n + 2))
+    tuned = None
+    if not args.no_tune:
+        from gcdt_amd import tune
+
+        def run_large():
+            solve_device(dp, n, dc, c, dt, da)
+
+        tuned = tune.load_or_tune("large", run_large)
+        print(f"# tunables (large): {tuned}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         T, st = solve_device(dp, n, dc, c, dt, da)
         print(f"# warmup {i}: {st['ms_total']:.1f} ms", file=sys.stderr, flush=True)
@@ -194,6 +205,7 @@ def main(argv=None):
                "sample": f"the full workload ({n} points, {c} segments), serial exact-predicate CDT, "
                          f"one host core, {dtc:.2f} s"}
 
+    degenerate = None if args.no_degenerate else bench_degenerate(args)
     batched = bench_batched(args) if args.batch_count > 0 else None
     serving = bench_serving(args) if args.serve_count > 0 else None
 
@@ -212,13 +224,15 @@ def main(argv=None):
         "data": "synthetic (seeded SplitMix64 generator)",
         "config": {"workload": "plain CDT, large: 10M int32 points (80% uniform, 20% Gaussian clusters "
                                "sigma=2^12, 64 centres) + 1000 disjoint 100-gon circles (100k segments)",
-                   "points": n, "segments": c, "triangles": T, "parallelism": "single GPU"},
+                   "points": n, "segments": c, "triangles": T, "parallelism": "single GPU",
+                   "tunables": tuned["choice"] if tuned else "defaults"},
         "roofline": roof,
         "solve_roofline": solve_roof,
         "kernels": kernels,
         "validity": valid,
         "last_step": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in steps[-1].items()},
         "cpu_baseline": cpu,
+        "degenerate": degenerate,
         "batched": batched,
         "serving": serving,
     }
@@ -246,6 +260,12 @@ def bench_batched(args):
     dp = DeviceBuffer.from_array(flat_p)
     dc = DeviceBuffer.from_array(flat_c)
     dt = DeviceBuffer(12 * int(toff[-1]))
+    tuned_b = tuned_s = None
+    if not args.no_tune:
+        from gcdt_amd import tune
+
+        tuned_b = tune.load_or_tune("batched", lambda: solve_batch_device(dp, poff, dc, coff, dt))
+        print(f"# tunables (batched): {tuned_b}", file=sys.stderr, flush=True)
     st = solve_batch_device(dp, poff, dc, coff, dt)  # warmup
     reps = 2
     check(L.gcdt_synchronize(), "sync")
@@ -273,6 +293,15 @@ def bench_batched(args):
     for g, c in inst[:m]:
         bufs.append((DeviceBuffer.from_array(g), DeviceBuffer.from_array(c) if len(c) else None,
                      DeviceBuffer(12 * (2 * len(g) + 2)), DeviceBuffer(12 * (2 * len(g) + 2)), len(g), len(c)))
+    if not args.no_tune:
+        from gcdt_amd import tune
+
+        def run_small():
+            for dpi, dci, dti, dai, nn, cc in bufs[:16]:
+                solve_device(dpi, nn, dci, cc, dti, dai)
+
+        tuned_s = tune.load_or_tune("per_instance", run_small)
+        print(f"# tunables (per-instance): {tuned_s}", file=sys.stderr, flush=True)
     solve_device(bufs[0][0], bufs[0][4], bufs[0][1], bufs[0][5], bufs[0][2], bufs[0][3])  # warmup
     check(L.gcdt_synchronize(), "sync")
     ts = time.perf_counter()
@@ -287,6 +316,11 @@ def bench_batched(args):
                 x.free()
     batched_cdts = len(inst) / (ms_batch / 1000.0)
     single_cdts = m / (ms_single / 1000.0)
+    cpu = None
+    if not args.no_cpu:
+        rate, thr, secs = _cpu_pool_cdts(inst)
+        cpu = {"value": round(rate, 1), "unit": "CDTs/s", "cores": thr, "kind": "port",
+               "sample": f"all {len(inst)} instances, one per host thread, serial exact-predicate CDT, {secs:.2f} s"}
     # the per-instance rate is measured on the first m instances; scale by point
     # counts too so an unrepresentative subset is visible
     return {"instances": len(inst), "points": int(len(flat_p)), "segments": int(len(flat_c)),
@@ -298,7 +332,62 @@ def bench_batched(args):
             "speedup_cdts": round(batched_cdts / single_cdts, 2),
             "rounds": {"insert": st["insert_rounds"], "flip": st["flip_rounds"],
                        "constraints": st["constraint_rounds"]},
-            "validated_instances": checked, "invalid_instances": bad}
+            "validated_instances": checked, "invalid_instances": bad, "cpu_baseline": cpu,
+            "tunables": {"batched": tuned_b["choice"] if tuned_b else "defaults",
+                         "per_instance": tuned_s["choice"] if tuned_s else "defaults"}}
+
+
+def _cpu_pool_cdts(instances, max_threads=16):
+    """The serial oracle over every instance, one instance per host thread
+    (the ctypes call releases the GIL).  Returns (CDTs/s, threads, seconds)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda gc: O.cdt(gc[0], gc[1]), instances))
+    dt = time.perf_counter() - t0
+    return len(instances) / dt, threads, dt
+
+
+def bench_degenerate(args):
+    """Degenerate stress: the 4096 x 4096 lattice (16.7M points, spacing 256)
+    plus 2,000 diagonal segments through collinear lattice points; one warmup
+    and one timed solve, then the exact validator (not unique: cocircular
+    everywhere, so validity is the check)."""
+    import numpy as np
+
+    from gcdt_amd import DeviceBuffer, gen, solve_device
+    from oracle import oracle as O
+
+    t0 = time.time()
+    pts, cons = gen.lattice_cdt()
+    g = gen.to_grid(pts)
+    n, c = len(g), len(cons)
+    print(f"# degenerate workload: {n} points, {c} segments (generated in {time.time() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    dp = DeviceBuffer.from_array(g)
+    dc = DeviceBuffer.from_array(cons)
+    dt = DeviceBuffer(12 * (2 * n + 2))
+    da = DeviceBuffer(12 * (2 * n + 2))
+    try:
+        solve_device(dp, n, dc, c, dt, da)
+        T, st = solve_device(dp, n, dc, c, dt, da)
+        tris = dt.to_array(np.int32, (T, 3))
+    finally:
+        for b in (dp, dc, dt, da):
+            b.free()
+    print(f"# degenerate solve: {st['ms_total']:.1f} ms ({st})", file=sys.stderr, flush=True)
+    v = O.validate(g, tris, cons)
+    print(f"# degenerate validation: {v}", file=sys.stderr, flush=True)
+    return {"workload": "4096x4096 lattice (spacing 256) + 2000 diagonal segments", "points": n, "segments": c,
+            "ms": round(st["ms_total"], 3), "mpts_per_s": round(n / (st["ms_total"] / 1000.0) / 1e6, 3),
+            "segments_after_split": st["segments"], "flips": st["flips"],
+            "rounds": {"insert": st["insert_rounds"], "flip": st["flip_rounds"], "constraints": st["constraint_rounds"]},
+            "valid": v["valid"], "not_delaunay": v["not_delaunay"], "missing_constraints": v["missing_constraints"],
+            "cocircular_edges": v["cocircular"]}
 
 
 def bench_serving(args):
@@ -306,6 +395,8 @@ def bench_serving(args):
     polygon boundaries of 1-5% of the points) arriving as a Poisson process at
     `serve_qps`; worker streams merge small requests into batched solves.
     Reports throughput and p50/p99 latency from arrival to result on device."""
+    import numpy as np
+
     from gcdt_amd import gen
     from gcdt_amd import serve as S
 
@@ -334,6 +425,31 @@ def bench_serving(args):
                  "mpts_per_s": round(npts / max(st.makespan_s, 1e-9) / 1e6, 3),
                  "validated_requests": checked, "invalid_requests": bad,
                  "latency": "arrival to triangles resident on the device"})
+    # saturation: every request waiting at t=0; continuous batching on the
+    # worker streams against one plain solve at a time on one stream
+    def fresh():
+        return [S.Request(r.rid, 0.0, r.pts, r.cons) for r in reqs]
+
+    sat_reqs = fresh()
+    sat = S.Server(workers=args.serve_workers).run(sat_reqs, realtime=False)
+    for r in sat_reqs:
+        r.fetch()
+    one_reqs = fresh()
+    one = S.Server(workers=1, small_n=0).run(one_reqs, realtime=False)
+    for r in one_reqs:
+        r.fetch()
+    sat_rate = sat.completed / max(sat.makespan_s, 1e-9)
+    one_rate = one.completed / max(one.makespan_s, 1e-9)
+    summ["saturated"] = {"cdts_per_s": round(sat_rate, 2), "mean_batch": round(float(np.mean(sat.batch_sizes)), 2),
+                         "batches": sat.batches, "p50_ms": sat.summary(0)["p50_ms"],
+                         "p99_ms": sat.summary(0)["p99_ms"],
+                         "one_at_a_time_cdts_per_s": round(one_rate, 2),
+                         "speedup_cdts": round(sat_rate / max(one_rate, 1e-9), 2)}
+    if not args.no_cpu:
+        rate, thr, secs = _cpu_pool_cdts([(r.pts, r.cons) for r in reqs])
+        summ["cpu_baseline"] = {"value": round(rate, 2), "unit": "CDTs/s", "cores": thr, "kind": "port",
+                                "sample": f"all {len(reqs)} requests, one per host thread, serial exact-predicate "
+                                          f"CDT, {secs:.2f} s"}
     print(f"# serving: {summ}", file=sys.stderr, flush=True)
     return summ
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -64,6 +65,19 @@ int gcdt_device_count(int* count)
     int n = 0;
     GCDT_HIP(hipGetDeviceCount(&n));
     if (count) *count = n;
+    return GCDT_OK;
+    API_END
+}
+
+int gcdt_device_name(char* buf, int len)
+{
+    API_BEGIN
+    if (!buf || len <= 0) return fail(GCDT_ERR_INVALID, "bad buffer");
+    int dev = 0;
+    GCDT_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    GCDT_HIP(hipGetDeviceProperties(&prop, dev));
+    snprintf(buf, (size_t)len, "%s:%s:cu%d", prop.name, prop.gcnArchName, prop.multiProcessorCount);
     return GCDT_OK;
     API_END
 }
@@ -805,10 +819,27 @@ extern "C" int gcdt_set_tunable(const char* name, int value)
 {
     if (!name) return fail(GCDT_ERR_INVALID, "null name");
     std::string n(name);
-    if (n == "tail_threshold") Tunables::get().tail_threshold = value;
-    else if (n == "pivot") Tunables::get().pivot = value;
-    else if (n == "single_tester") Tunables::get().single_tester = value;
-    else return fail(GCDT_ERR_INVALID, "unknown tunable " + n);
+    auto range = [&](int lo, int hi) {
+        return value >= lo && value <= hi ? GCDT_OK
+                                          : fail(GCDT_ERR_INVALID, n + " must lie in [" + std::to_string(lo) + ", " +
+                                                                       std::to_string(hi) + "]");
+    };
+    int rc;
+    if (n == "tail_threshold") {
+        if ((rc = range(0, 1 << 20))) return rc;
+        Tunables::get().tail_threshold = value;
+    } else if (n == "pivot") {
+        if ((rc = range(0, 1))) return rc;
+        Tunables::get().pivot = value;
+    } else if (n == "single_tester") {
+        if ((rc = range(0, 1))) return rc;
+        Tunables::get().single_tester = value;
+    } else if (n == "grid_cap") {
+        if ((rc = range(256, 65536))) return rc;
+        g_grid_cap = value;
+    } else {
+        return fail(GCDT_ERR_INVALID, "unknown tunable " + n);
+    }
     return GCDT_OK;
 }
 
@@ -819,6 +850,7 @@ extern "C" int gcdt_get_tunable(const char* name, int* value)
     if (n == "tail_threshold") *value = Tunables::get().tail_threshold;
     else if (n == "pivot") *value = Tunables::get().pivot;
     else if (n == "single_tester") *value = Tunables::get().single_tester;
+    else if (n == "grid_cap") *value = g_grid_cap;
     else return fail(GCDT_ERR_INVALID, "unknown tunable " + n);
     return GCDT_OK;
 }

@@ -32,11 +32,14 @@ struct Error : std::runtime_error {
 constexpr int kBlock = 256;         // 4 wave64 per workgroup
 constexpr int kMaxGrid = 256 * 16;  // persistent grid-stride cap: 16 WGs per CU
 
+// Cap on workgroups of a grid-stride launch (a tunable; kMaxGrid by default).
+extern int g_grid_cap;
+
 inline int grid_for(long long n)
 {
     long long g = (n + kBlock - 1) / kBlock;
     if (g < 1) g = 1;
-    if (g > kMaxGrid) g = kMaxGrid;
+    if (g > g_grid_cap) g = g_grid_cap;
     return (int)g;
 }
 

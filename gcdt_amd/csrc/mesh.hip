@@ -22,12 +22,14 @@ Tunables& Tunables::get()
         if (const char* e = getenv("GCDT_TAIL_THRESHOLD")) x.tail_threshold = atoi(e);
         if (const char* e = getenv("GCDT_PIVOT")) x.pivot = atoi(e);
         if (const char* e = getenv("GCDT_SINGLE_TESTER")) x.single_tester = atoi(e);
+        if (const char* e = getenv("GCDT_GRID_CAP")) g_grid_cap = std::max(256, std::min(65536, atoi(e)));
         return x;
     }();
     return t;
 }
 
 thread_local std::vector<PendingFree>* g_defer = nullptr;
+int g_grid_cap = kMaxGrid;
 
 namespace {
 // Pinned counter mirrors and ring events, reused across solves: allocating

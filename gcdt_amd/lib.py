@@ -65,6 +65,7 @@ SIGNATURES = {
     "gcdt_last_error": ([], ctypes.c_char_p),
     "gcdt_device_count": ([pint], i32),
     "gcdt_synchronize": ([], i32),
+    "gcdt_device_name": ([ctypes.c_char_p, i32], i32),
     "gcdt_alloc": ([pvp, sz], i32),
     "gcdt_release": ([vp, sz], i32),
     "gcdt_pool_init": ([], i32),

@@ -258,17 +258,20 @@ class Server:
 
     def run(self, requests, realtime: bool = True):
         """Feed `requests` at their arrival times (seconds, relative) and serve
-        them to completion.  Returns the stats object."""
+        them to completion; with realtime=False all of them are waiting at
+        the start.  Returns the stats object."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         self.t0 = self.clock()
         threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(self.workers)]
+        if not realtime:  # everything is waiting before the first worker looks
+            with self.cv:
+                self.waiting.extend(reqs)
         for th in threads:
             th.start()
-        for r in reqs:
-            if realtime:
-                dt = r.arrival - self.now()
-                if dt > 0:
-                    time.sleep(dt)
+        for r in reqs if realtime else ():
+            dt = r.arrival - self.now()
+            if dt > 0:
+                time.sleep(dt)
             with self.cv:
                 self.waiting.append(r)
                 self.cv.notify_all()

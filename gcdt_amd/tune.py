@@ -1,0 +1,121 @@
+"""Autotuner for the solve's queue and launch sizes.
+
+Two knobs decide how the round loops map onto the GPU:
+
+* ``tail_threshold`` -- list length at or below which the remaining rounds
+  run in one persistent 1024-thread workgroup instead of one launch per phase;
+* ``grid_cap`` -- workgroups of a grid-stride launch (items per thread).
+
+``tune`` times a workload over a small grid of values by coordinate descent
+(threshold first, then the grid cap), keeps the fastest, applies it with
+gcdt_set_tunable and caches it per device and workload class in a JSON file
+(``$GCDT_TUNE_CACHE`` or ``~/.cache/gcdt_amd/tune.json``); ``load_or_tune``
+applies a cached choice without timing.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import time
+
+from .lib import check, lib
+
+CANDIDATES = {
+    "tail_threshold": [1024, 2048, 4096, 8192, 16384],
+    "grid_cap": [2048, 4096, 8192, 16384],
+}
+DEFAULTS = {"tail_threshold": 4096, "grid_cap": 4096}
+
+
+def cache_path() -> str:
+    p = os.environ.get("GCDT_TUNE_CACHE")
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "gcdt_amd", "tune.json")
+
+
+def device_key() -> str:
+    buf = ctypes.create_string_buffer(256)
+    check(lib().gcdt_device_name(buf, 256), "gcdt_device_name")
+    return buf.value.decode()
+
+
+def get(name: str) -> int:
+    v = ctypes.c_int(0)
+    check(lib().gcdt_get_tunable(name.encode(), ctypes.byref(v)), "gcdt_get_tunable")
+    return v.value
+
+
+def apply(choice: dict):
+    for k, v in choice.items():
+        check(lib().gcdt_set_tunable(k.encode(), int(v)), "gcdt_set_tunable")
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path, data):
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only loses the cache
+
+
+def _time(run, repeats):
+    best = float("inf")
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        run()
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def tune(kind: str, run, candidates=None, repeats: int = 2, key: str | None = None, path: str | None = None):
+    """Time `run()` (one complete solve, synchronised) over the candidates and
+    apply the fastest.  Returns {"choice": {...}, "ms": best, "trials": [...]}."""
+    candidates = candidates or CANDIDATES
+    choice = {k: get(k) for k in candidates}
+    trials = []
+    run()  # warm code objects and the pool at the current setting
+    for name, values in candidates.items():
+        best_v, best_t = choice[name], None
+        for v in values:
+            trial = dict(choice, **{name: v})
+            apply(trial)
+            t = _time(run, repeats)
+            trials.append({**trial, "ms": round(t * 1000.0, 3)})
+            if best_t is None or t < best_t:
+                best_v, best_t = v, t
+        choice[name] = best_v
+    apply(choice)
+    best_ms = min(t["ms"] for t in trials if all(t[k] == choice[k] for k in choice))
+    res = {"choice": choice, "ms": best_ms, "trials": trials}
+    path = path or cache_path()
+    data = _load(path)
+    data.setdefault(key or device_key(), {})[kind] = {"choice": choice, "ms": best_ms}
+    _store(path, data)
+    return res
+
+
+def load_or_tune(kind: str, run, force: bool = False, **kw):
+    """Apply the cached choice for (device, kind), or tune and cache one."""
+    path = kw.pop("path", None) or cache_path()
+    key = kw.pop("key", None) or device_key()
+    if not force:
+        hit = _load(path).get(key, {}).get(kind)
+        if hit:
+            apply(hit["choice"])
+            return {"choice": hit["choice"], "ms": hit.get("ms"), "cached": True}
+    res = tune(kind, run, key=key, path=path, **kw)
+    res["cached"] = False
+    return res

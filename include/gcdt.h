@@ -52,6 +52,8 @@ const char* gcdt_last_error(void);
 /* Number of visible devices; selects device 0 on first use. */
 int gcdt_device_count(int* count);
 int gcdt_synchronize(void);
+/* "<name>:<gfx arch>:cu<compute units>" of the current device (autotuner cache key). */
+int gcdt_device_name(char* buf, int len);
 
 /* MemCache pool (reference: MemCache.cpp:7-30,87-99 getMemByAbtrtSize /
  * releaseAbtrtMem; include/utils.cuh:33-39,65-68 utils::malloc/release). */
@@ -215,9 +217,13 @@ typedef struct {
 int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int K, const gcdt_int2* cons,
                      const int64_t* cons_offsets, gcdt_int3* tris_out, gcdt_batch_stats* stats, void* stream);
 
-/* process-wide tunables read by every later solve: "tail_threshold"
- * (list length below which rounds finish in one persistent workgroup),
- * "pivot" (0 random, 1 central-first), "single_tester" (0/1). */
+/* Process-wide tunables read by every later solve (set them between solves,
+ * not while a solve runs on another thread): "tail_threshold" [0, 2^20] (list
+ * length at or below which rounds finish in one persistent workgroup),
+ * "grid_cap" [256, 65536] (workgroups of a grid-stride launch), "pivot"
+ * (0 random, 1 central-first), "single_tester" (0/1).  Out-of-range values
+ * fail with GCDT_ERR_INVALID.  gcdt_amd/tune.py picks tail_threshold and
+ * grid_cap by timing and caches the choice per device. */
 int gcdt_set_tunable(const char* name, int value);
 int gcdt_get_tunable(const char* name, int* value);
 

@@ -17,7 +17,13 @@ def test_serving_results_match_alone_and_oracle():
     srv = S.Server(workers=3, small_n=8000, batch_points=40_000)
     st = srv.run(reqs)
     assert st.completed == len(reqs) and st.failed == 0, [r.error for r in reqs if r.error]
-    assert max(st.batch_sizes) > 1            # some requests were merged
+    # all waiting at once: small requests are merged into batched solves
+    queued = [S.Request(r.rid, 0.0, r.pts, r.cons) for r in reqs]
+    st2 = S.Server(workers=3, small_n=8000, batch_points=40_000).run(queued, realtime=False)
+    assert st2.completed == len(reqs) and max(st2.batch_sizes) > 1
+    for r, q in zip(reqs, queued):
+        assert np.array_equal(O.canonical(r.fetch()), O.canonical(q.fetch())) or not O.validate(
+            r.pts, r.fetch(), r.cons)["unique"]
     for r in reqs:
         tris = r.fetch()
         assert tris.shape == (2 * r.n + 2, 3)
