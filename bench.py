@@ -186,6 +186,18 @@ def main(argv=None):
     solve_roof = {"bytes": b_alg, "achieved_GBs": round(b_alg / (ms_per_step / 1000.0) / 1e9, 3),
                   "frac": round(b_alg / (ms_per_step / 1000.0) / 1e9 / HBM_PEAK_GBS, 6)}
 
+    import ctypes as _ct
+    in_use, cached, peak = _ct.c_size_t(0), _ct.c_size_t(0), _ct.c_size_t(0)
+    check(L.gcdt_pool_stats(_ct.byref(in_use), _ct.byref(cached), _ct.byref(peak)), "pool")
+    pool_peak = int(peak.value)
+    # the same solve from host arrays (PCIe in and out): reported, never `value`
+    from gcdt_amd import solve as solve_host
+    th = time.perf_counter()
+    solve_host(g, cons)
+    ms_host = (time.perf_counter() - th) * 1000.0
+    pcie = {"ms": round(ms_host, 3), "mpts_per_s": round(n / (ms_host / 1000.0) / 1e6, 3),
+            "note": "host points/segments in, host triangles out (gcdt_amd.solve)"}
+
     valid = None
     if not args.no_validate:
         from oracle import oracle as O
@@ -228,6 +240,8 @@ def main(argv=None):
                    "tunables": tuned["choice"] if tuned else "defaults"},
         "roofline": roof,
         "solve_roofline": solve_roof,
+        "pcie_inclusive": pcie,
+        "pool_peak_bytes": pool_peak,
         "kernels": kernels,
         "validity": valid,
         "last_step": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in steps[-1].items()},
