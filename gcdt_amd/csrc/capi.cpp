@@ -214,6 +214,12 @@ int gcdt_prof_enable(int on)
     return GCDT_OK;
 }
 
+int gcdt_prof_only(const char* kernel)
+{
+    prof_only(kernel);
+    return GCDT_OK;
+}
+
 int gcdt_prof_reset(void)
 {
     prof_reset();
@@ -256,14 +262,16 @@ static void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, i
     DBuf<int> perm(N > 0 ? N : 1), inv(N > 0 ? N : 1);
     DBuf<int2> cons_buf(C > 0 ? C : 1);
     int2* pts = pts_buf.p;
-    morton_order(c, pts_in, N, pts, perm.p, inv.p);
+    DBuf<int> cell_end((size_t)(1 << 20) + 1);
+    morton_order(c, pts_in, N, pts, perm.p, inv.p, cell_end.p);
     relabel_pairs(c, cons_in, C, inv.p, N, cons_buf.p);
     const int2* cons = cons_buf.p;
     write_corners(c, pts, N);
     // ---- 1. sample + sample mesh
     Timer t1(c.st);
     DBuf<int> sid(1 << 16);
-    int S = sample_points(c, pts, N, sid.p, nullptr);
+    int S = sample_points_morton(c, cell_end.p, sid.p);
+    cell_end.release();
     DBuf<int> loc((size_t)N + 4), pend((size_t)(N > 1 ? N : 1)), pend2((size_t)(N > 1 ? N : 1));
     DBuf<unsigned long long> pkey((size_t)N + 4);
     long long capA = 4LL * S + 8;

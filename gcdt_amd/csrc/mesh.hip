@@ -148,13 +148,26 @@ struct ProfRec {
 };
 std::mutex g_prof_mu;
 bool g_prof_on = false;
+std::string g_prof_only;            // time only this kernel when not empty
 std::vector<ProfRec> g_prof;
+std::vector<hipEvent_t> g_ev_pool;  // events are reused across resets
+
+bool take_event(hipEvent_t* e)
+{
+    if (!g_ev_pool.empty()) {
+        *e = g_ev_pool.back();
+        g_ev_pool.pop_back();
+        return true;
+    }
+    return hipEventCreate(e) == hipSuccess;
+}
 }  // namespace
 
 ProfScope::ProfScope(const char* name, hipStream_t s, long long items) : st(s)
 {
     if (!g_prof_on) return;
     std::lock_guard<std::mutex> g(g_prof_mu);
+    if (!g_prof_only.empty() && g_prof_only != name) return;
     for (size_t i = 0; i < g_prof.size(); i++)
         if (g_prof[i].name == name) slot = (int)i;
     if (slot < 0) {
@@ -162,7 +175,12 @@ ProfScope::ProfScope(const char* name, hipStream_t s, long long items) : st(s)
         slot = (int)g_prof.size() - 1;
     }
     hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    if (!take_event(&a)) {
+        slot = -1;
+        return;
+    }
+    if (!take_event(&b)) {
+        g_ev_pool.push_back(a);
         slot = -1;
         return;
     }
@@ -190,10 +208,17 @@ void prof_reset()
     std::lock_guard<std::mutex> g(g_prof_mu);
     for (auto& r : g_prof)
         for (auto& e : r.ev) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
+            (void)hipEventSynchronize(e.second);
+            g_ev_pool.push_back(e.first);
+            g_ev_pool.push_back(e.second);
         }
     g_prof.clear();
+}
+
+void prof_only(const char* name)
+{
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof_only = name ? name : "";
 }
 
 // total ms, launches and items of one kernel (synchronises its events)
@@ -542,17 +567,49 @@ __global__ __launch_bounds__(kBlock) void k_morton_scatter(const int2* pts, int 
     }
 }
 
-void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv)
+void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv, int* cell_end)
 {
+    fill_i32(c, cell_end, 0, BITMAP_CELLS + 1);
     if (n <= 0) return;
-    DBuf<int> cnt(BITMAP_CELLS + 1);
-    fill_i32(c, cnt.p, 0, BITMAP_CELLS + 1);
-    k_morton_count<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cnt.p);
+    k_morton_count<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end);
     GCDT_LAUNCH_CHECK();
-    scan_excl(c, cnt.p, cnt.p, BITMAP_CELLS);
-    k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cnt.p, out, perm, inv);
+    scan_excl(c, cell_end, cell_end, BITMAP_CELLS);
+    // the scatter advances each cell's cursor to the cell's end
+    k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, out, perm, inv);
     GCDT_LAUNCH_CHECK();
-    c.get(C_TOTAL);  // keeps cnt alive until the scatter has run
+}
+
+// Sample of Morton-ordered points: the 256x256 sample cell with Morton index
+// m covers the 16 consecutive 20-bit cells [16m, 16m+16), i.e. the sorted
+// range [end[16m-1], end[16m+15]); its first point is the cell's sample.
+__device__ inline int sample_range_start(const int* cell_end, int m)
+{
+    return m ? cell_end[16 * m - 1] : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sample_morton_flags(const int* cell_end, int* flags)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < SAMPLE_CELLS; i += gridDim.x * blockDim.x)
+        flags[i] = cell_end[16 * i + 15] > sample_range_start(cell_end, i);
+}
+
+__global__ __launch_bounds__(kBlock) void k_sample_morton_emit(const int* cell_end, const int* offs, int* ids)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < SAMPLE_CELLS; i += gridDim.x * blockDim.x) {
+        int a = sample_range_start(cell_end, i);
+        if (cell_end[16 * i + 15] > a) ids[offs[i]] = a;
+    }
+}
+
+int sample_points_morton(Ctx& c, const int* cell_end, int* sample_ids)
+{
+    DBuf<int> offs(SAMPLE_CELLS + 1);
+    k_sample_morton_flags<<<grid_for(SAMPLE_CELLS), kBlock, 0, c.st>>>(cell_end, offs.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl(c, offs.p, offs.p, SAMPLE_CELLS);
+    k_sample_morton_emit<<<grid_for(SAMPLE_CELLS), kBlock, 0, c.st>>>(cell_end, offs.p, sample_ids);
+    GCDT_LAUNCH_CHECK();
+    return c.get(C_TOTAL);
 }
 
 __global__ __launch_bounds__(kBlock) void k_relabel_pairs(const int2* in, int C, const int* inv, int n, int2* out)

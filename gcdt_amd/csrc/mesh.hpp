@@ -161,6 +161,7 @@ struct ProfScope {
 };
 
 void prof_enable(bool on);
+void prof_only(const char* name);  // restrict timing to one kernel (NULL: all)
 void prof_reset();
 bool prof_read(const char* name, double* ms, long long* launches, long long* items);
 int prof_names(char* buf, int cap);
@@ -188,7 +189,12 @@ int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int n
 long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds);
 // spatial order: points sorted by the Morton code of their 1024^2 cell
 // (out[k] = pts[perm[k]], inv = perm^-1)
-void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv);
+// Counting sort of points by 20-bit Morton cell; cell_end[BITMAP_CELLS + 1]
+// receives each cell's end offset in the sorted order.
+void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv, int* cell_end);
+// One sample point per occupied 256x256 cell of Morton-ordered points (the
+// first of the cell's range), from morton_order's cell_end; returns the count.
+int sample_points_morton(Ctx& c, const int* cell_end, int* sample_ids);
 void relabel_pairs(Ctx& c, const int2* in, int C, const int* inv, int n, int2* out);
 void unmap_tris(Ctx& c, int3* tri, int T, const int* perm, int n);
 // batches: instance map, per-instance corners/seed triangles/location, local

@@ -88,6 +88,7 @@ SIGNATURES = {
     "gcdt_write_mesh": ([ctypes.c_char_p, vp, i32, vp, ctypes.c_int64], i32),
     "gcdt_prof_enable": ([i32], i32),
     "gcdt_prof_reset": ([], i32),
+    "gcdt_prof_only": ([ctypes.c_char_p], i32),
     "gcdt_prof_read": ([ctypes.c_char_p, pdbl, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)], i32),
     "gcdt_prof_names": ([ctypes.c_char_p, i32], i32),
     "gcdt_get_sample_points": ([vp, sz, vp, i32, pvp, pvp, pint, pdbl], i32),

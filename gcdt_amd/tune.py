@@ -80,23 +80,32 @@ def _time(run, repeats):
     return best
 
 
-def tune(kind: str, run, candidates=None, repeats: int = 2, key: str | None = None, path: str | None = None):
+def tune(kind: str, run, candidates=None, repeats: int = 3, key: str | None = None, path: str | None = None,
+         min_gain: float = 0.02):
     """Time `run()` (one complete solve, synchronised) over the candidates and
-    apply the fastest.  Returns {"choice": {...}, "ms": best, "trials": [...]}."""
+    apply the fastest; a knob leaves its current value only for a candidate
+    at least `min_gain` faster (run-to-run noise is ~1-2%).  Returns
+    {"choice": {...}, "ms": best, "trials": [...]}."""
     candidates = candidates or CANDIDATES
     choice = {k: get(k) for k in candidates}
     trials = []
     run()  # warm code objects and the pool at the current setting
     for name, values in candidates.items():
-        best_v, best_t = choice[name], None
+        times = {}
         for v in values:
             trial = dict(choice, **{name: v})
             apply(trial)
             t = _time(run, repeats)
             trials.append({**trial, "ms": round(t * 1000.0, 3)})
-            if best_t is None or t < best_t:
-                best_v, best_t = v, t
-        choice[name] = best_v
+            times[v] = t
+        cur = choice[name]
+        if cur not in times:
+            apply(choice)
+            times[cur] = _time(run, repeats)
+            trials.append({**choice, "ms": round(times[cur] * 1000.0, 3)})
+        best_v = min(times, key=times.get)
+        if times[best_v] < (1.0 - min_gain) * times[cur]:
+            choice[name] = best_v
     apply(choice)
     best_ms = min(t["ms"] for t in trials if all(t[k] == choice[k] for k in choice))
     res = {"choice": choice, "ms": best_ms, "trials": trials}

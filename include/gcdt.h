@@ -85,6 +85,9 @@ size_t gcdt_scratch_bytes(int64_t n);
  * ';'-separated names seen so far and returns the full length. */
 int gcdt_prof_enable(int on);
 int gcdt_prof_reset(void);
+/* Time only the named kernel (NULL or "": all): fewer event markers in the
+ * stream when one kernel's live duration is wanted inside a timed region. */
+int gcdt_prof_only(const char* kernel);
 int gcdt_prof_read(const char* kernel, double* total_ms, int64_t* launches, int64_t* items);
 int gcdt_prof_names(char* buf, int cap);
 
