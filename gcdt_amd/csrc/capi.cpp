@@ -845,6 +845,9 @@ extern "C" int gcdt_set_tunable(const char* name, int value)
     } else if (n == "grid_cap") {
         if ((rc = range(256, 65536))) return rc;
         g_grid_cap = value;
+    } else if (n == "flip_items" || n == "ins_items") {
+        if (value != 1 && value != 2 && value != 4) return fail(GCDT_ERR_INVALID, n + " must be 1, 2 or 4");
+        (n == "flip_items" ? Tunables::get().flip_items : Tunables::get().ins_items) = value;
     } else {
         return fail(GCDT_ERR_INVALID, "unknown tunable " + n);
     }
@@ -859,6 +862,8 @@ extern "C" int gcdt_get_tunable(const char* name, int* value)
     else if (n == "pivot") *value = Tunables::get().pivot;
     else if (n == "single_tester") *value = Tunables::get().single_tester;
     else if (n == "grid_cap") *value = g_grid_cap;
+    else if (n == "flip_items") *value = Tunables::get().flip_items;
+    else if (n == "ins_items") *value = Tunables::get().ins_items;
     else return fail(GCDT_ERR_INVALID, "unknown tunable " + n);
     return GCDT_OK;
 }

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mesh.hpp"
@@ -56,7 +57,7 @@ HostSlot take_host_slot()
     }
     HostSlot h{};
     GCDT_HIP(hipHostMalloc((void**)&h.hcnt, sizeof(int) * C_NUM, hipHostMallocDefault));
-    GCDT_HIP(hipHostMalloc((void**)&h.hring, sizeof(int) * 16, hipHostMallocDefault));
+    GCDT_HIP(hipHostMalloc((void**)&h.hring, sizeof(int) * 2 * C_NUM, hipHostMallocDefault));
     GCDT_HIP(hipEventCreateWithFlags(&h.ev[0], hipEventDisableTiming));
     GCDT_HIP(hipEventCreateWithFlags(&h.ev[1], hipEventDisableTiming));
     return h;
@@ -68,6 +69,8 @@ Ctx::Ctx(hipStream_t s) : st(s)
     tail_threshold = Tunables::get().tail_threshold;
     pivot = Tunables::get().pivot;
     single_tester = Tunables::get().single_tester;
+    flip_items = Tunables::get().flip_items;
+    ins_items = Tunables::get().ins_items;
     dcnt.alloc(C_NUM);
     HostSlot h = take_host_slot();
     hcnt = h.hcnt;
@@ -98,14 +101,14 @@ Ctx::~Ctx()
 
 void Ctx::snapshot(int slot)
 {
-    GCDT_HIP(hipMemcpyAsync(hring + 8 * slot, dcnt.p, sizeof(int) * 8, hipMemcpyDeviceToHost, st));
+    GCDT_HIP(hipMemcpyAsync(hring + C_NUM * slot, dcnt.p, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, st));
     GCDT_HIP(hipEventRecord(ring_ev[slot], st));
 }
 
 const int* Ctx::wait_snapshot(int slot)
 {
     GCDT_HIP(hipEventSynchronize(ring_ev[slot]));
-    return hring + 8 * slot;
+    return hring + C_NUM * slot;
 }
 
 void Ctx::read(int i, int k)
@@ -213,6 +216,14 @@ void prof_reset()
             g_ev_pool.push_back(e.second);
         }
     g_prof.clear();
+}
+
+void prof_add_items(const char* name, long long items)
+{
+    if (!g_prof_on) return;
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (auto& r : g_prof)
+        if (r.name == name) r.items += items;
 }
 
 void prof_only(const char* name)
@@ -796,8 +807,17 @@ __global__ __launch_bounds__(kBlock) void k_ins_claim(Ins s, const int* list, co
     }
 }
 
-constexpr int INS_ITEMS = 4;
+// Items per thread of the block-appending list kernels: 1, 2 or 4 (tunable;
+// more items amortise the block-wide append, fewer keep more waves in flight).
+template <class F>
+void with_items(int v, F&& f)
+{
+    if (v == 1) f(std::integral_constant<int, 1>{});
+    else if (v == 2) f(std::integral_constant<int, 2>{});
+    else f(std::integral_constant<int, 4>{});
+}
 
+template <int INS_ITEMS>
 __global__ __launch_bounds__(kBlock) void k_ins_split(Ins s, const int* list, const int* cnt, unsigned epoch,
                                                       int* ntri)
 {
@@ -832,6 +852,7 @@ __global__ __launch_bounds__(kBlock) void k_ins_fix(Ins s, const int* cnt)
         ins_fix(s, (int)i);
 }
 
+template <int INS_ITEMS>
 __global__ __launch_bounds__(kBlock) void k_ins_reloc(Ins s, const int* list, const int* cnt, int* list2, int* cnt2)
 {
     int n = *cnt;
@@ -934,6 +955,10 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
         if (h[C_ERR]) c.check_err();
         if (h[C_NPEND] > 0 && h[C_NPEND2] >= h[C_NPEND] && h[C_NTRI] == h[C_TBASE])
             throw Error(GCDT_ERR_INTERNAL, "insertion round without progress");
+        // exact list lengths of the round, for per-item bytes of the timed kernels
+        prof_add_items("ins_claim", h[C_NPEND]);
+        prof_add_items("ins_split", h[C_NPEND]);
+        prof_add_items("ins_reloc", h[C_NPEND]);
     };
     while (!done && est > c.tail_threshold) {
         if (++r > 100000) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion rounds did not converge");
@@ -941,27 +966,30 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
         k_round_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, r > 1);
         GCDT_LAUNCH_CHECK();
         int g = grid_for(est);
-        int g4 = grid_for((est + INS_ITEMS - 1) / INS_ITEMS);
         {
-            ProfScope ps("ins_claim", c.st, est);
+            ProfScope ps("ins_claim", c.st, 0);
             k_ins_claim<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, ep);
             GCDT_LAUNCH_CHECK();
         }
-        {
-            ProfScope ps("ins_split", c.st, est);
-            k_ins_split<<<g4, kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, ep, c.dcnt.p + C_NTRI);
+        with_items(c.ins_items, [&](auto it) {
+            constexpr int IT = decltype(it)::value;
+            ProfScope ps("ins_split", c.st, 0);
+            k_ins_split<IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, ep,
+                                                                                 c.dcnt.p + C_NTRI);
             GCDT_LAUNCH_CHECK();
-        }
+        });
         {
             ProfScope ps("ins_fix", c.st, 0);
             k_ins_fix<<<grid_for(4LL * est), kBlock, 0, c.st>>>(s, c.dcnt.p);
             GCDT_LAUNCH_CHECK();
         }
-        {
-            ProfScope ps("ins_reloc", c.st, est);
-            k_ins_reloc<<<g4, kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, P2, c.dcnt.p + C_NPEND2);
+        with_items(c.ins_items, [&](auto it) {
+            constexpr int IT = decltype(it)::value;
+            ProfScope ps("ins_reloc", c.st, 0);
+            k_ins_reloc<IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, P2,
+                                                                                 c.dcnt.p + C_NPEND2);
             GCDT_LAUNCH_CHECK();
-        }
+        });
         c.snapshot(r & 1);
         std::swap(P, P2);
         if (r >= 2) {
@@ -1008,6 +1036,7 @@ __global__ void k_flip_begin(int* cnt, int move)
     if (move) cnt[C_NACT] = cnt[C_NACT2];
     cnt[C_NACT2] = 0;
     cnt[C_NFLIP] = 0;
+    cnt[C_NPROP] = 0;
 }
 
 // Flip engines: the array-per-field layout (short lists on a mesh whose
@@ -1016,7 +1045,7 @@ __global__ void k_flip_begin(int* cnt, int move)
 // two triangles of a flip.
 struct EngSoA {
     Flip f;
-    __device__ void test(int t, unsigned ep, bool all, bool single) const { flip_test(f, t, ep, all, single); }
+    __device__ bool test(int t, unsigned ep, bool all, bool single) const { return flip_test(f, t, ep, all, single); }
     template <bool COH>
     __device__ int apply(int t, unsigned ep, int* nn) const
     {
@@ -1033,7 +1062,10 @@ struct EngSoA {
 
 struct EngPacked {
     FlipP f;
-    __device__ void test(int t, unsigned ep, bool all, bool single) const { flip_test_p(f, t, ep, all, single); }
+    __device__ bool test(int t, unsigned ep, bool all, bool single) const
+    {
+        return flip_test_p(f, t, ep, all, single);
+    }
     template <bool COH>
     __device__ int apply(int t, unsigned ep, int* nn) const
     {
@@ -1043,17 +1075,35 @@ struct EngPacked {
     __device__ void set_mark(int t, int v) const { f.r[t].mark = v; }
 };
 
-template <class E>
-__global__ __launch_bounds__(kBlock) void k_flip_test(E e, const int* list, const int* cnt, unsigned epoch, int single)
+// Tests the listed triangles; the ones that propose a flip are appended to
+// `prop`, the only list the apply step reads.
+template <class E, int FT_ITEMS>
+__global__ __launch_bounds__(kBlock) void k_flip_test(E e, const int* list, const int* cnt, unsigned epoch, int single,
+                                                      int* prop, int* nprop)
 {
     int n = *cnt;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        e.test(list ? list[i] : (int)i, epoch, list == nullptr, single != 0);
+    TILE_LOOP(base, n, kBlock * FT_ITEMS)
+    {
+        int q[FT_ITEMS];
+        int nq = 0;
+#pragma unroll
+        for (int j = 0; j < FT_ITEMS; j++) {
+            q[j] = -1;
+            long long i = base + (long long)j * kBlock + threadIdx.x;
+            if (i < n) {
+                int t = list ? list[i] : (int)i;
+                if (e.test(t, epoch, list == nullptr, single != 0)) q[j] = t;
+            }
+            nq += q[j] >= 0;
+        }
+        int b = block_alloc<kBlock>(nq, nprop);
+#pragma unroll
+        for (int j = 0; j < FT_ITEMS; j++)
+            if (q[j] >= 0) prop[b++] = q[j];
+    }
 }
 
-constexpr int FA_ITEMS = 4;  // list items per thread per block-wide append
-
-template <class E>
+template <class E, int FA_ITEMS>
 __global__ __launch_bounds__(kBlock) void k_flip_apply(E e, const int* list, const int* cnt, unsigned epoch, int* list2,
                                                        int* cnt2, int* flipped, int* nflip)
 {
@@ -1068,7 +1118,7 @@ __global__ __launch_bounds__(kBlock) void k_flip_apply(E e, const int* list, con
             qb[j] = -1;
             long long i = base + (long long)j * kBlock + threadIdx.x;
             if (i < n) {
-                int t = list ? list[i] : (int)i;
+                int t = list[i];
                 int nn = -1;
                 int res = e.template apply<false>(t, epoch, &nn);
                 if (res == 1) e.set_mark(t, (int)epoch + 1);
@@ -1209,7 +1259,7 @@ namespace {
 template <class E>
 long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int* rounds)
 {
-    DBuf<int> la(T), lb(T), lf(T);
+    DBuf<int> la(T), lb(T), lf(T), lp(T);
     const int* act = init;
     int* nxt = la.p;
     int* other = lb.p;
@@ -1224,11 +1274,16 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
     int r = 0;
     int est = init ? ninit : T;
     bool done = est == 0;
+    int checked = 0;  // rounds whose snapshot has been read
     auto check_round = [&](const int* h) {
         if (h[C_ERR]) c.check_err();
         // the highest-keyed proposal always wins both of its triangles
         if (h[C_NACT2] > 0 && h[C_NFLIP] == 0) throw Error(GCDT_ERR_INTERNAL, "flip round without progress");
         flips += h[C_NFLIP] / 2;
+        ++checked;
+        prof_add_items("flip_test", (checked == 1 && !init) ? h[C_CONST] : h[C_NACT]);
+        prof_add_items("flip_apply", h[C_NPROP]);
+        prof_add_items("flip_fix", h[C_NFLIP]);
     };
     while (!done && est > c.tail_threshold) {
         if (++r > 1000000) throw Error(GCDT_ERR_NOT_CONVERGED, "flip rounds did not converge");
@@ -1236,18 +1291,21 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         int cnt_idx = (r == 1 && !init) ? C_CONST : C_NACT;
         k_flip_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, r > 1);
         GCDT_LAUNCH_CHECK();
-        int g = grid_for(est);
-        {
-            ProfScope ps("flip_test", c.st, est);
-            k_flip_test<E><<<g, kBlock, 0, c.st>>>(e, act, c.dcnt.p + cnt_idx, ep, c.single_tester);
-            GCDT_LAUNCH_CHECK();
-        }
-        {
-            ProfScope ps("flip_apply", c.st, est);
-            k_flip_apply<E><<<grid_for((est + FA_ITEMS - 1) / FA_ITEMS), kBlock, 0, c.st>>>(
-                e, act, c.dcnt.p + cnt_idx, ep, nxt, c.dcnt.p + C_NACT2, lf.p, c.dcnt.p + C_NFLIP);
-            GCDT_LAUNCH_CHECK();
-        }
+        with_items(c.flip_items, [&](auto it) {
+            constexpr int IT = decltype(it)::value;
+            {
+                ProfScope ps("flip_test", c.st, 0);
+                k_flip_test<E, IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(
+                    e, act, c.dcnt.p + cnt_idx, ep, c.single_tester, lp.p, c.dcnt.p + C_NPROP);
+                GCDT_LAUNCH_CHECK();
+            }
+            {
+                ProfScope ps("flip_apply", c.st, 0);
+                k_flip_apply<E, IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(
+                    e, lp.p, c.dcnt.p + C_NPROP, ep, nxt, c.dcnt.p + C_NACT2, lf.p, c.dcnt.p + C_NFLIP);
+                GCDT_LAUNCH_CHECK();
+            }
+        });
         {
             ProfScope ps("flip_fix", c.st, 0);
             k_flip_fix<E><<<grid_for(est), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, ep);

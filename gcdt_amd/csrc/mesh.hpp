@@ -97,6 +97,7 @@ enum : int {
     C_TAIL_ERR = 18,  // 1 = a round without progress
     C_TAIL_PAR = 19,  // parity: the final list is the second buffer
     C_TAIL_F = 20,    // flips done
+    C_NPROP = 21,     // proposals of the current flip round
     C_NUM = 32
 };
 
@@ -106,6 +107,8 @@ struct Tunables {
     int tail_threshold = 4096;
     int pivot = 0;
     int single_tester = 1;
+    int flip_items = 1;  // items per thread of flip test/apply: 1, 2 or 4
+    int ins_items = 1;   // items per thread of insertion split/relocate: 1, 2 or 4
     static Tunables& get();
 };
 
@@ -125,6 +128,8 @@ struct Ctx {
     // and whether both-active edges are tested by the lower triangle only
     int pivot = 0;
     int single_tester = 1;
+    int flip_items = 1;
+    int ins_items = 1;
     explicit Ctx(hipStream_t s);
     ~Ctx();
     Ctx(const Ctx&) = delete;
@@ -138,10 +143,10 @@ struct Ctx {
         return hcnt[i];
     }
     void set(int i, int v);
-    // pipelined snapshots of counters [0, 8): ring of 2 pinned slots + events
+    // pipelined snapshots of all counters: ring of 2 pinned slots + events
     int* hring = nullptr;
     hipEvent_t ring_ev[2] = {nullptr, nullptr};
-    void snapshot(int slot);          // async copy of counters 0..7 + event
+    void snapshot(int slot);          // async copy of the counters + event
     const int* wait_snapshot(int slot);  // waits for that copy
     void zero_counters();
     void check_err();  // throws on a device error flag
@@ -162,6 +167,9 @@ struct ProfScope {
 
 void prof_enable(bool on);
 void prof_only(const char* name);  // restrict timing to one kernel (NULL: all)
+// items processed by a timed kernel, known only after its launch (exact list
+// lengths read back from the round counters)
+void prof_add_items(const char* name, long long items);
 void prof_reset();
 bool prof_read(const char* name, double* ms, long long* launches, long long* items);
 int prof_names(char* buf, int cap);

@@ -305,7 +305,7 @@ __host__ __device__ inline int opposite_vertex(const Mesh& m, int n, int u, int 
 // (all_active: every triangle is active this epoch).  Every illegal edge keeps
 // an active triangle: the lower one proposes an edge, so it is re-queued or
 // flipped.
-__host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch, bool all_active, bool single = true)
+__host__ __device__ inline bool flip_test(const Flip& f, int t, unsigned epoch, bool all_active, bool single = true)
 {
     const Mesh& m = f.m;
     int3 tv = m.tri[t];
@@ -328,6 +328,7 @@ __host__ __device__ inline void flip_test(const Flip& f, int t, unsigned epoch, 
         }
     }
     f.cand[t] = cand;
+    return cand >= 0;
 }
 
 // Flip t's proposed edge if it won both triangles.  Returns 0 no proposal,
@@ -466,8 +467,8 @@ struct FlipP {
     int* err;
 };
 
-// flip_test over records (same rule as flip_test).
-__host__ __device__ inline void flip_test_p(const FlipP& f, int t, unsigned epoch, bool all_active, bool single = true)
+// flip_test over records (same rule as flip_test); true if t proposed.
+__host__ __device__ inline bool flip_test_p(const FlipP& f, int t, unsigned epoch, bool all_active, bool single = true)
 {
     const FRec r = rec_load(&f.r[t]);
     const int cf = r.meta & 7;
@@ -490,6 +491,7 @@ __host__ __device__ inline void flip_test_p(const FlipP& f, int t, unsigned epoc
         }
     }
     f.r[t].meta = cf | (cand << 3);
+    return cand != 3;
 }
 
 // flip_apply over records: same claims protocol and return codes.  A flip

@@ -4,10 +4,13 @@ Two knobs decide how the round loops map onto the GPU:
 
 * ``tail_threshold`` -- list length at or below which the remaining rounds
   run in one persistent 1024-thread workgroup instead of one launch per phase;
-* ``grid_cap`` -- workgroups of a grid-stride launch (items per thread).
+* ``grid_cap`` -- workgroups of a grid-stride launch;
+* ``flip_items`` / ``ins_items`` -- list items per thread of the kernels that
+  append to a list once per block (more items amortise the append, fewer keep
+  more waves in flight).
 
 ``tune`` times a workload over a small grid of values by coordinate descent
-(threshold first, then the grid cap), keeps the fastest, applies it with
+(one knob at a time, in the order above), keeps the fastest, applies it with
 gcdt_set_tunable and caches it per device and workload class in a JSON file
 (``$GCDT_TUNE_CACHE`` or ``~/.cache/gcdt_amd/tune.json``); ``load_or_tune``
 applies a cached choice without timing.
@@ -24,8 +27,10 @@ from .lib import check, lib
 CANDIDATES = {
     "tail_threshold": [1024, 2048, 4096, 8192, 16384],
     "grid_cap": [2048, 4096, 8192, 16384],
+    "flip_items": [1, 2, 4],
+    "ins_items": [1, 2, 4],
 }
-DEFAULTS = {"tail_threshold": 4096, "grid_cap": 4096}
+DEFAULTS = {"tail_threshold": 4096, "grid_cap": 4096, "flip_items": 1, "ins_items": 1}
 
 
 def cache_path() -> str:

@@ -223,10 +223,13 @@ int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int 
 /* Process-wide tunables read by every later solve (set them between solves,
  * not while a solve runs on another thread): "tail_threshold" [0, 2^20] (list
  * length at or below which rounds finish in one persistent workgroup),
- * "grid_cap" [256, 65536] (workgroups of a grid-stride launch), "pivot"
+ * "grid_cap" [256, 65536] (workgroups of a grid-stride launch),
+ * "flip_items" / "ins_items" (1, 2 or 4 list items per thread of the flip
+ * test/apply and insertion split/relocate launches), "pivot"
  * (0 random, 1 central-first), "single_tester" (0/1).  Out-of-range values
- * fail with GCDT_ERR_INVALID.  gcdt_amd/tune.py picks tail_threshold and
- * grid_cap by timing and caches the choice per device. */
+ * fail with GCDT_ERR_INVALID.  gcdt_amd/tune.py picks tail_threshold,
+ * grid_cap and the items per thread by timing and caches the choice per
+ * device. */
 int gcdt_set_tunable(const char* name, int value);
 int gcdt_get_tunable(const char* name, int* value);
 

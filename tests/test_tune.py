@@ -12,7 +12,7 @@ from gcdt_amd import tune
 
 @pytest.fixture(autouse=True)
 def _restore():
-    saved = {k: get_tunable(k) for k in ("tail_threshold", "grid_cap")}
+    saved = {k: get_tunable(k) for k in tune.CANDIDATES}
     yield
     tune.apply(saved)
 
@@ -29,7 +29,8 @@ def _fake_run(calls):
 def test_tuner_finds_minimum_and_caches(tmp_path):
     path = str(tmp_path / "tune.json")
     calls = []
-    res = tune.tune("unit", _fake_run(calls), repeats=1, key="cpu-test", path=path)
+    knobs = {"tail_threshold": tune.CANDIDATES["tail_threshold"], "grid_cap": tune.CANDIDATES["grid_cap"]}
+    res = tune.tune("unit", _fake_run(calls), candidates=knobs, repeats=1, key="cpu-test", path=path)
     assert res["choice"] == {"tail_threshold": 8192, "grid_cap": 16384}
     assert get_tunable("tail_threshold") == 8192 and get_tunable("grid_cap") == 16384
     data = json.load(open(path))
@@ -49,3 +50,7 @@ def test_tunable_ranges_are_checked():
         set_tunable("no_such_knob", 1)
     set_tunable("grid_cap", 8192)
     assert get_tunable("grid_cap") == 8192
+    with pytest.raises(GcdtError, match="1, 2 or 4"):
+        set_tunable("flip_items", 3)
+    set_tunable("flip_items", 2)
+    assert get_tunable("flip_items") == 2
