@@ -1072,7 +1072,7 @@ struct EngPacked {
         return flip_apply_p<COH>(f, t, ep, nn);
     }
     __device__ void fix(int t, unsigned ep) const { flip_fix_p(f, t, ep); }
-    __device__ void set_mark(int t, int v) const { f.r[t].mark = v; }
+    __device__ void set_mark(int t, int v) const { f.r[t].meta = rec_with_mark(f.r[t].meta, v); }
 };
 
 // Tests the listed triangles; the ones that propose a flip are appended to
@@ -1229,11 +1229,8 @@ __global__ __launch_bounds__(kBlock) void k_pack_faces(const int3* __restrict__ 
         x.p[0] = pts[x.v.x];
         x.p[1] = pts[x.v.y];
         x.p[2] = pts[x.v.z];
-        x.meta = (cflag ? cflag[t] & 7 : 0) | FREC_NOCAND;
-        x.mark = 0;
-        x.stamp = 0;
-        x.partner = -1;
-        r[t] = x;
+        rec_store(&r[t], x.v, x.a, x.p[0], x.p[1], x.p[2], (cflag ? cflag[t] & 7 : 0) | FREC_NOCAND | FREC_NOPARTNER,
+                  0, 0ull);
     }
 }
 
@@ -1241,7 +1238,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack_faces(const FRec* __restrict_
                                                          int3* __restrict__ adj, int* __restrict__ cflag)
 {
     for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < T; t += (long long)gridDim.x * blockDim.x) {
-        const FRec x = r[t];
+        const FRec x = rec_load(&r[t]);
         tri[t] = x.v;
         adj[t] = x.a;
         if (cflag) cflag[t] = x.meta & 7;
@@ -1367,8 +1364,6 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
         if (rounds) *rounds = 0;
         return 0;
     }
-    DBuf<unsigned long long> claim(T);
-    fill_u64(c, claim.p, 0, T);
     if (!init) {
         // whole mesh: packed records for the rounds, written back at the end
         DBuf<FRec> rec(T);
@@ -1378,7 +1373,7 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
             GCDT_LAUNCH_CHECK();
         }
         EngPacked e;
-        e.f = FlipP{rec.p, claim.p, c.dcnt.p + C_ERR};
+        e.f = FlipP{rec.p, c.dcnt.p + C_ERR};
         long long flips = flip_driver(c, e, T, nullptr, 0, rounds);
         {
             ProfScope ps("flip_unpack", c.st, T);
@@ -1387,6 +1382,8 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
         }
         return flips;
     }
+    DBuf<unsigned long long> claim(T);
+    fill_u64(c, claim.p, 0, T);
     DBuf<int> stamp(T), partner(T), cand(T), mark(T);
     fill_i32(c, stamp.p, 0, T);
     fill_i32(c, mark.p, 0, T);

@@ -407,22 +407,34 @@ __host__ __device__ inline void flip_fix(const Flip& f, int t, unsigned epoch)
 // ---------------------------------------------------------------- packed flip records
 
 // One 64-byte record per triangle for flip rounds over a whole mesh.  A test
-// or a flip reads a neighbour's vertices, neighbours, vertex coordinates and
-// flip bookkeeping from one record -- one cache-line visit -- where the
-// array-per-field layout (tri, adj, pts, cflag, cand, mark, stamp, partner)
-// gathers from up to eight places.
+// or a flip reads a neighbour's vertices, neighbours, vertex coordinates,
+// flip bookkeeping and claim word from one record -- one line -- where the
+// array-per-field layout gathers from up to eight places.
 struct FRec {
-    int3 v;       // vertices, counter-clockwise
-    int3 a;       // neighbour across the edge opposite v[k]; marked while a flip's outer edges settle
-    int2 p[3];    // coordinates of v[0..2]
-    int meta;     // bits 0-2: constrained edges (bit k = edge opposite v[k]); bits 3-4: proposed edge, 3 = none
-    int mark;     // epoch in which the triangle is in the active list
-    int stamp;    // epoch of its last flip
-    int partner;  // the other triangle of that flip
+    int3 v;        // vertices, counter-clockwise
+    int3 a;        // neighbour across the edge opposite v[k]; marked while a flip's outer edges settle
+    int2 p[3];     // coordinates of v[0..2]
+    int meta;      // bits 0-2 constrained edges (bit k = edge opposite v[k]); 3-4 proposed edge (3 none);
+                   // 5-6 slot of the last flip's partner (3 none); 7-31 epoch in which it is active
+    int stamp;     // epoch of its last flip
+    unsigned long long claim;  // highest proposal key of the round (atomicMax)
 };
 static_assert(sizeof(FRec) == 64, "flip record is one 64-byte line (arrays of records start 256-byte aligned)");
 
 constexpr int FREC_NOCAND = 3 << 3;
+constexpr int FREC_NOPARTNER = 3 << 5;
+constexpr unsigned FREC_MARK_MASK = 0x1FFFFFFu;  // 25-bit epochs (a solve runs far fewer rounds)
+
+__host__ __device__ inline int rec_cand(int meta) { return (meta >> 3) & 3; }
+__host__ __device__ inline int rec_pslot(int meta) { return (meta >> 5) & 3; }
+__host__ __device__ inline bool rec_marked(int meta, unsigned epoch)
+{
+    return ((unsigned)meta >> 7) == (epoch & FREC_MARK_MASK);
+}
+__host__ __device__ inline int rec_with_mark(int meta, int epoch)
+{
+    return (meta & 0x7F) | (int)(((unsigned)epoch & FREC_MARK_MASK) << 7);
+}
 
 // Whole-record load and store as four 16-byte accesses (a struct copy of an
 // over-aligned local is otherwise staged through the stack).
@@ -437,20 +449,19 @@ __host__ __device__ inline FRec rec_load(const FRec* p)
     r.p[1] = make_int2(c.x, c.y);
     r.p[2] = make_int2(c.z, c.w);
     r.meta = d.x;
-    r.mark = d.y;
-    r.stamp = d.z;
-    r.partner = d.w;
+    r.stamp = d.y;
+    r.claim = (unsigned long long)(unsigned)d.z | ((unsigned long long)(unsigned)d.w << 32);
     return r;
 }
 
 __host__ __device__ inline void rec_store(FRec* p, const int3& v, const int3& a, const int2& p0, const int2& p1,
-                                          const int2& p2, int meta, int mark, int stamp, int partner)
+                                          const int2& p2, int meta, int stamp, unsigned long long claim)
 {
     int4* q = reinterpret_cast<int4*>(p);
     q[0] = make_int4(v.x, v.y, v.z, a.x);
     q[1] = make_int4(a.y, a.z, p0.x, p0.y);
     q[2] = make_int4(p1.x, p1.y, p2.x, p2.y);
-    q[3] = make_int4(meta, mark, stamp, partner);
+    q[3] = make_int4(meta, stamp, (int)(unsigned)claim, (int)(unsigned)(claim >> 32));
 }
 
 // coordinate k of a record by selects over values (a runtime index into the
@@ -463,7 +474,6 @@ __host__ __device__ inline int2 sel_pt(int2 p0, int2 p1, int2 p2, int k)
 
 struct FlipP {
     FRec* r;
-    unsigned long long* claim;
     int* err;
 };
 
@@ -472,43 +482,50 @@ __host__ __device__ inline bool flip_test_p(const FlipP& f, int t, unsigned epoc
 {
     const FRec r = rec_load(&f.r[t]);
     const int cf = r.meta & 7;
+    const int ps = rec_pslot(r.meta);
+    const int partner = (r.stamp == (int)epoch - 1 && ps < 3) ? get3(r.a, ps) : -1;
     int cand = 3;
     for (int k = 0; k < 3; k++) {
         if ((cf >> k) & 1) continue;
         int n = get3(r.a, k);
         if (n < 0) continue;
+        // the diagonal of last round's flip is locally Delaunay (it replaced a
+        // strictly illegal edge, and neither triangle has changed since)
+        if (n == partner) continue;
         bool lower = single && n < t;
         if (lower && all_active) continue;
         const FRec rn = rec_load(&f.r[n]);
-        if (lower && rn.mark == (int)epoch) continue;
+        if (lower && rec_marked(rn.meta, epoch)) continue;
         int j = slot_other(rn.v, get3(r.v, nx3(k)), get3(r.v, pv3(k)));
         if (incircle(r.p[0], r.p[1], r.p[2], rec_pt(rn, j)) > 0) {
             cand = k;
             unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
-            at_max(&f.claim[t], key);
-            at_max(&f.claim[n], key);
+            at_max(&f.r[t].claim, key);
+            at_max(&f.r[n].claim, key);
             break;
         }
     }
-    f.r[t].meta = cf | (cand << 3);
+    f.r[t].meta = (r.meta & ~(3 << 3)) | (cand << 3);
     return cand != 3;
 }
 
 // flip_apply over records: same claims protocol and return codes.  A flip
-// rewrites both records whole, including mark = epoch + 1 (both are queued).
+// rewrites both records whole -- keeping the claim word, which losers may be
+// reading, at the winning key -- with the new vertices, coordinates,
+// constraint bits, partner slot and mark = epoch + 1 (both are queued).
 template <bool COH = true>
 __host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoch, int* nout)
 {
-    const int k = (f.r[t].meta >> 3) & 3;
+    const int k = rec_cand(f.r[t].meta);
     if (k == 3) return 0;
     const FRec r = rec_load(&f.r[t]);
     const int n = get3(r.a, k);
     const unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
-    const unsigned long long ct = ld_claim<COH>(&f.claim[t]);
-    if (ct != key || ld_claim<COH>(&f.claim[n]) != key) {
+    const unsigned long long ct = COH ? ld_claim<COH>(&f.r[t].claim) : r.claim;
+    if (ct != key || ld_claim<COH>(&f.r[n].claim) != key) {
         if (ct != key) {
             int t2 = (int)((ct & 0xffffffffu) / 3);
-            if ((ct >> 32) == epoch && ld_claim<COH>(&f.claim[t2]) == ct) return 3;
+            if ((ct >> 32) == epoch && ld_claim<COH>(&f.r[t2].claim) == ct) return 3;
         }
         return 1;
     }
@@ -520,11 +537,14 @@ __host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoc
     const int o = get3(rn.v, j);
     const int cft = r.meta & 7, cfn = rn.meta & 7;
     const int2 pp = rec_pt(r, k), pu = rec_pt(r, kn), pv = rec_pt(r, kp), po = rec_pt(rn, j);
-    // t = (p,u,v), n = (o,v,u)  ->  t = (p,u,o), n = (p,o,v)
+    const int mk = (int)epoch + 1;
+    // t = (p,u,v), n = (o,v,u)  ->  t = (p,u,o) [partner in slot 1], n = (p,o,v) [partner in slot 2]
     rec_store(&f.r[t], make_int3(p, u, o), make_int3(adj_mark(get3(rn.a, jn)), n, adj_mark(get3(r.a, kp))), pp, pu, po,
-              ((cfn >> jn) & 1) | (((cft >> kp) & 1) << 2) | FREC_NOCAND, (int)epoch + 1, (int)epoch, n);
+              rec_with_mark(((cfn >> jn) & 1) | (((cft >> kp) & 1) << 2) | FREC_NOCAND | (1 << 5), mk), (int)epoch,
+              key);
     rec_store(&f.r[n], make_int3(p, o, v), make_int3(adj_mark(get3(rn.a, jp)), adj_mark(get3(r.a, kn)), t), pp, po, pv,
-              ((cfn >> jp) & 1) | (((cft >> kn) & 1) << 1) | FREC_NOCAND, (int)epoch + 1, (int)epoch, t);
+              rec_with_mark(((cfn >> jp) & 1) | (((cft >> kn) & 1) << 1) | FREC_NOCAND | (2 << 5), mk), (int)epoch,
+              key);
     *nout = n;
     return 2;
 }
@@ -541,7 +561,7 @@ __host__ __device__ inline void flip_fix_p(const FlipP& f, int t, unsigned epoch
         int u = get3(r.v, nx3(k)), v = get3(r.v, pv3(k));
         const FRec rb = rec_load(&f.r[nb]);
         if (rb.stamp == (int)epoch) {
-            set3(ta, k, has2(rb.v, u, v) ? nb : rb.partner);
+            set3(ta, k, has2(rb.v, u, v) ? nb : get3(rb.a, rec_pslot(rb.meta)));
         } else {
             set3(ta, k, nb);
             (&f.r[nb].a.x)[slot_other(rb.v, u, v)] = t;
