@@ -84,6 +84,33 @@ def solve(pts_grid: np.ndarray, cons: np.ndarray | None = None, with_adj: bool =
     return (tris, st, adj) if with_adj else (tris, st)
 
 
+def solve_carved(pts_grid: np.ndarray, cons: np.ndarray):
+    """CDT plus even-odd carving.  Returns (all triangles (2n+2,3), constraint
+    bits (2n+2,), inside mask (2n+2,) bool, stats)."""
+    p = np.ascontiguousarray(pts_grid, dtype=np.int32).reshape(-1, 2)
+    n = len(p)
+    c = np.ascontiguousarray(cons, dtype=np.int32).reshape(-1, 2)
+    dp = DeviceBuffer.from_array(p)
+    dc = DeviceBuffer.from_array(c) if len(c) else None
+    T = 2 * n + 2
+    dt, da, df, di = DeviceBuffer(12 * T), DeviceBuffer(12 * T), DeviceBuffer(4 * T), DeviceBuffer(4 * T)
+    try:
+        nt = ctypes.c_int(0)
+        st = Stats()
+        check(lib().gcdt_solve_flags(dp.ptr, n, dc.ptr if dc is not None else None, len(c), dt.ptr, da.ptr, df.ptr,
+                                     ctypes.byref(nt), ctypes.byref(st), None), "gcdt_solve_flags")
+        ni = ctypes.c_int(0)
+        check(lib().gcdt_carve(dt.ptr, da.ptr, df.ptr, nt.value, n, di.ptr, ctypes.byref(ni), None), "gcdt_carve")
+        tris = dt.to_array(np.int32, (nt.value, 3))
+        cflag = df.to_array(np.int32, (nt.value,))
+        inside = di.to_array(np.int32, (nt.value,)).astype(bool)
+    finally:
+        for b in (dp, dc, dt, da, df, di):
+            if b is not None:
+                b.free()
+    return tris, cflag, inside, st.as_dict()
+
+
 def pack_instances(instances):
     """[(pts_grid (n_i,2), cons (c_i,2) local ids)] -> flat arrays + offsets."""
     pts = [np.ascontiguousarray(p, dtype=np.int32).reshape(-1, 2) for p, _ in instances]

@@ -252,7 +252,7 @@ struct SolveOut {
 
 // Full pipeline on device arrays; pts holds N+4 entries (corners written here).
 static void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, int3* tris_out,
-                       int3* adj_out, SolveOut& out)
+                       int3* adj_out, SolveOut& out, int* cflag_out = nullptr)
 {
     gcdt_stats& st = out.st;
     Timer total(c.st);
@@ -347,11 +347,17 @@ static void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, i
     st.ms_flip = t4.ms();
     // ---- 5. constraints
     Timer t5(c.st);
+    if (cflag_out) fill_i32(c, cflag_out, 0, T);
     if (C > 0) {
-        DBuf<int> cflag(T), touched(T);
-        fill_i32(c, cflag.p, 0, T);
+        DBuf<int> cflag_buf, touched(T);
+        int* cflag = cflag_out;
+        if (!cflag) {
+            cflag_buf.alloc(T);
+            cflag = cflag_buf.p;
+            fill_i32(c, cflag, 0, T);
+        }
         fill_i32(c, touched.p, 0, T);
-        Mesh m{pts, tris_out, adj_out, nullptr, cflag.p};
+        Mesh m{pts, tris_out, adj_out, nullptr, cflag};
         ConsResult cr = insert_constraints(c, m, T, N + 4, N, cons, C, touched.p, nullptr);
         st.constraint_rounds = cr.rounds;
         st.segments = cr.pieces;
@@ -391,6 +397,22 @@ extern "C" int gcdt_solve(const gcdt_int2* points, int N, const gcdt_int2* cons,
     }
     SolveOut out;
     solve_impl(c, (const int2*)points, N, (const int2*)cons, C, (int3*)tris_out, adj, out);
+    if (num_tris) *num_tris = out.T;
+    if (stats) *stats = out.st;
+    return GCDT_OK;
+    API_END
+}
+
+extern "C" int gcdt_solve_flags(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcdt_int3* tris_out,
+                                gcdt_int3* adj_out, int* cflag_out, int* num_tris, gcdt_stats* stats, void* stream)
+{
+    API_BEGIN
+    if (N < 0 || C < 0 || (N > 0 && !points) || (C > 0 && !cons) || !tris_out || !adj_out || !cflag_out)
+        return fail(GCDT_ERR_INVALID, "bad arguments");
+    if (N > (1 << 28)) return fail(GCDT_ERR_INVALID, "too many points");
+    Ctx c((hipStream_t)stream);
+    SolveOut out;
+    solve_impl(c, (const int2*)points, N, (const int2*)cons, C, (int3*)tris_out, (int3*)adj_out, out, cflag_out);
     if (num_tris) *num_tris = out.T;
     if (stats) *stats = out.st;
     return GCDT_OK;

@@ -191,6 +191,21 @@ typedef struct {
 int gcdt_solve(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcdt_int3* tris_out,
                gcdt_int3* adj_out, int* num_tris, gcdt_stats* stats, void* stream);
 
+/* gcdt_solve that also writes the constraint bits of every output triangle
+ * (bit k of cflag_out[t] set: the edge opposite vertex k lies on a
+ * constraint); adj_out and cflag_out are required. */
+int gcdt_solve_flags(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcdt_int3* tris_out,
+                     gcdt_int3* adj_out, int* cflag_out, int* num_tris, gcdt_stats* stats, void* stream);
+
+/* Even-odd carving (no counterpart in the reference, which keeps every
+ * triangle of the bounding square): inside[t] = 1 for the triangles enclosed
+ * by an odd number of constraint boundaries -- polygon interiors minus their
+ * holes -- counted from the region holding the square's corners (vertex ids
+ * >= numPoints).  Open polylines separate nothing and change no parity.
+ * tris/adj/cflag as written by gcdt_solve_flags; all arrays on the device. */
+int gcdt_carve(const gcdt_int3* tris, const gcdt_int3* adj, const int* cflag, int T, int numPoints, int* inside,
+               int* num_inside, void* stream);
+
 /* ------------------------------------------------------------ batches */
 
 typedef struct {

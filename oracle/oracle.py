@@ -12,6 +12,9 @@ Functions:
   brute_force_dt(pts_grid)       O(n^4) empty-circumcircle enumeration on
                                  tiny inputs (Python big ints), pins cdt()
   orient2d / incircle            scalar predicates for golden-vector checks
+  constraint_bits / carve_even_odd / doubled_area / loops / loop_doubled_area
+                                 even-odd carving by 0-1 breadth-first search
+                                 over the triangles, and exact areas that pin it
 """
 from __future__ import annotations
 
@@ -201,3 +204,115 @@ def brute_force_cdt(pts_grid, cons) -> np.ndarray:
         if not bad:
             out.append(tri)
     return canonical(np.array(out, dtype=np.int64).reshape(-1, 3))
+
+
+# ---------------------------------------------------------------- even-odd carving
+
+def constraint_bits(pts_grid, tris, cons) -> np.ndarray:
+    """Bit k of the result for triangle t: the edge opposite vertex k lies on
+    one of the constraint segments (both endpoints on the closed segment,
+    decided with exact integer orientation).  O(edges x segments)."""
+    T = np.asarray(tris, dtype=np.int64).reshape(-1, 3)
+    Pi = with_corners(pts_grid).astype(np.int64)
+    bits = np.zeros(len(T), np.int32)
+    cons = np.asarray(cons, dtype=np.int64).reshape(-1, 2)
+    for k in range(3):
+        u = Pi[T[:, (k + 1) % 3]]
+        v = Pi[T[:, (k + 2) % 3]]
+        on = np.zeros(len(T), bool)
+        for a_id, b_id in cons:
+            a, b = Pi[a_id], Pi[b_id]
+            dx, dy = b[0] - a[0], b[1] - a[1]
+            # exact: |coordinates| < 2^31, products < 2^62
+            cu = dx * (u[:, 1] - a[1]) - dy * (u[:, 0] - a[0])
+            cv = dx * (v[:, 1] - a[1]) - dy * (v[:, 0] - a[0])
+            lo_x, hi_x = min(a[0], b[0]), max(a[0], b[0])
+            lo_y, hi_y = min(a[1], b[1]), max(a[1], b[1])
+            inside = ((u[:, 0] >= lo_x) & (u[:, 0] <= hi_x) & (u[:, 1] >= lo_y) & (u[:, 1] <= hi_y)
+                      & (v[:, 0] >= lo_x) & (v[:, 0] <= hi_x) & (v[:, 1] >= lo_y) & (v[:, 1] <= hi_y))
+            on |= (cu == 0) & (cv == 0) & inside
+        bits |= on.astype(np.int32) << k
+    return bits
+
+
+def carve_even_odd(tris, cbits, n) -> np.ndarray:
+    """inside[t]: the fewest constraint edges crossed from a triangle holding
+    a corner of the square (vertex id >= n) is odd.  0-1 breadth-first search
+    over the triangle adjacency (crossing a constrained edge costs 1)."""
+    import collections
+    T = np.asarray(tris, dtype=np.int64).reshape(-1, 3)
+    edge = {}
+    nbr = [[-1, -1, -1] for _ in range(len(T))]
+    for t, (a, b, c) in enumerate(T.tolist()):
+        for k, (u, v) in enumerate(((b, c), (c, a), (a, b))):
+            key = (min(u, v), max(u, v))
+            if key in edge:
+                s, j = edge.pop(key)
+                nbr[t][k] = s
+                nbr[s][j] = t
+            else:
+                edge[key] = (t, k)
+    INF = 1 << 60
+    dist = [INF] * len(T)
+    dq = collections.deque()
+    for t, row in enumerate(T.tolist()):
+        if max(row) >= n:
+            dist[t] = 0
+            dq.append(t)
+    while dq:
+        t = dq.popleft()
+        for k in range(3):
+            s = nbr[t][k]
+            if s < 0:
+                continue
+            w = (int(cbits[t]) >> k) & 1
+            if dist[t] + w < dist[s]:
+                dist[s] = dist[t] + w
+                (dq.append if w else dq.appendleft)(s)
+    return np.array([d % 2 == 1 for d in dist], bool)
+
+
+def doubled_area(pts_grid, tris) -> int:
+    """Twice the total signed area of the triangles (exact)."""
+    P = with_corners(pts_grid)
+    tot = 0
+    for a, b, c in np.asarray(tris).reshape(-1, 3).tolist():
+        ax, ay = int(P[a][0]), int(P[a][1])
+        tot += (int(P[b][0]) - ax) * (int(P[c][1]) - ay) - (int(P[b][1]) - ay) * (int(P[c][0]) - ax)
+    return tot
+
+
+def loops(cons):
+    """Closed vertex cycles of a constraint set whose vertices all have degree 2."""
+    nxt = {}
+    for a, b in np.asarray(cons).reshape(-1, 2).tolist():
+        nxt.setdefault(a, []).append(b)
+        nxt.setdefault(b, []).append(a)
+    seen = set()
+    out = []
+    for s in nxt:
+        if s in seen:
+            continue
+        cyc = [s]
+        seen.add(s)
+        prev, cur = None, s
+        while True:
+            cand = [w for w in nxt[cur] if w != prev]
+            w = cand[0]
+            if w == s:
+                break
+            cyc.append(w)
+            seen.add(w)
+            prev, cur = cur, w
+        out.append(cyc)
+    return out
+
+
+def loop_doubled_area(pts_grid, cyc) -> int:
+    P = np.asarray(pts_grid)
+    tot = 0
+    for i in range(len(cyc)):
+        x0, y0 = int(P[cyc[i]][0]), int(P[cyc[i]][1])
+        x1, y1 = int(P[cyc[(i + 1) % len(cyc)]][0]), int(P[cyc[(i + 1) % len(cyc)]][1])
+        tot += x0 * y1 - x1 * y0
+    return abs(tot)
