@@ -85,3 +85,21 @@ def test_rejects_out_of_range():
     g = np.array([[0, 5], [9, 9]], dtype=np.int32)
     with pytest.raises(GcdtError):
         _solve(g, np.zeros((0, 2), np.int32))
+
+
+def test_rejects_crossing_constraints():
+    """Two segments crossing at a non-vertex: the intersection is not a grid
+    point, so the solve reports GCDT_ERR_CROSSING instead of inventing one."""
+    from gcdt_amd import GcdtError
+    rng = np.random.default_rng(1)
+    p = np.concatenate([rng.integers(0, 1 << 20, size=(300, 2)),
+                        [[1000, 1000], [900000, 900000], [1000, 900000], [900000, 1000]]])
+    p, _ = gen.dedup(p)
+    idx = {tuple(x): i for i, x in enumerate(p.tolist())}
+    a, b, c, d = idx[(1000, 1000)], idx[(900000, 900000)], idx[(1000, 900000)], idx[(900000, 1000)]
+    g = gen.to_grid(p)
+    with pytest.raises(GcdtError) as e:
+        _solve(g, np.array([[a, b], [c, d]], np.int32))
+    assert e.value.status == 5
+    tris, _ = _solve(g, np.array([[a, b]], np.int32))   # one of them alone is fine
+    assert O.validate(g, tris, np.array([[a, b]]))["valid"]
