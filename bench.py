@@ -92,6 +92,8 @@ def main(argv=None):
     ap.add_argument("--serve-count", type=int, default=128, help="requests in the serving run (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving run")
     ap.add_argument("--serve-workers", type=int, default=3, help="worker streams of the serving run")
+    ap.add_argument("--serve-small-n", type=int, default=65536,
+                    help="requests up to this many points are merged into batched solves")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every GPU box here has one MI355X, and this CDT path has no "
@@ -434,7 +436,7 @@ def bench_serving(args):
     # warm the pool and the code objects on a few requests, outside the timed run
     S.Server(workers=args.serve_workers).run([S.Request(-1 - i, 0.0, r.pts, r.cons) for i, r in
                                               enumerate(reqs[:4])], realtime=False)
-    srv = S.Server(workers=args.serve_workers)
+    srv = S.Server(workers=args.serve_workers, small_n=args.serve_small_n)
     st = srv.run(reqs)
     summ = st.summary(args.serve_qps)
     from oracle import oracle as O
@@ -456,7 +458,7 @@ def bench_serving(args):
         return [S.Request(r.rid, 0.0, r.pts, r.cons) for r in reqs]
 
     sat_reqs = fresh()
-    sat = S.Server(workers=args.serve_workers).run(sat_reqs, realtime=False)
+    sat = S.Server(workers=args.serve_workers, small_n=args.serve_small_n).run(sat_reqs, realtime=False)
     for r in sat_reqs:
         r.fetch()
     one_reqs = fresh()

@@ -103,3 +103,48 @@ def test_rejects_crossing_constraints():
     assert e.value.status == 5
     tris, _ = _solve(g, np.array([[a, b]], np.int32))   # one of them alone is fine
     assert O.validate(g, tris, np.array([[a, b]]))["valid"]
+
+
+def test_empty_input():
+    tris, st = _solve(np.zeros((0, 2), np.int32), np.zeros((0, 2), np.int32))
+    assert len(tris) == 2 and O.validate(np.zeros((0, 2), np.int32), tris)["valid"]
+
+
+def test_collinear_points():
+    x = np.arange(500, dtype=np.int64) * 1000 + 7
+    p = np.stack([x, x // 2 + 11], 1)
+    _check(gen.to_grid(p), np.zeros((0, 2), np.int32))
+
+
+def test_extreme_grid_coordinates():
+    """Points one unit inside the square (the largest inCircle terms) used
+    directly on the 2^30 grid, with and without constraints."""
+    R = (1 << 30) - 1
+    ex = np.array([[1, 1], [R - 1, R - 1], [1, R - 1], [R - 1, 1], [1 << 29, 1 << 29], [1, 1 << 29],
+                   [R - 1, 1 << 29], [1 << 29, 1], [1 << 29, R - 1], [2, 3], [R - 2, R - 3]], np.int32)
+    rng = np.random.default_rng(9)
+    ex = np.concatenate([ex, rng.integers(1, R - 1, size=(2000, 2)).astype(np.int32)])
+    ex = np.unique(ex, axis=0)
+    _check(ex, np.zeros((0, 2), np.int32))
+    idx = {tuple(v): i for i, v in enumerate(ex.tolist())}
+    cons = np.array([[idx[(1, 1)], idx[(1, R - 1)]], [idx[(1, R - 1)], idx[(R - 1, R - 1)]]], np.int32)
+    _check(ex, cons)
+
+
+def test_duplicate_and_reversed_constraints():
+    rng = np.random.default_rng(3)
+    p, _ = gen.dedup(rng.integers(0, 1 << 20, size=(3000, 2)))
+    g = gen.to_grid(p)
+    cons = np.array([[10, 20], [20, 10], [10, 20], [20, 30], [30, 10]], np.int32)
+    tris, st = _solve(g, cons)
+    assert O.validate(g, tris, cons)["valid"]
+
+
+@pytest.mark.parametrize("bad", [[[5, 5]], [[0, 10 ** 6]], [[-1, 3]]])
+def test_rejects_bad_constraint_ids(bad):
+    from gcdt_amd import GcdtError
+    rng = np.random.default_rng(4)
+    p, _ = gen.dedup(rng.integers(0, 1 << 20, size=(200, 2)))
+    with pytest.raises(GcdtError) as e:
+        _solve(gen.to_grid(p), np.array(bad, np.int32))
+    assert e.value.status == 2
