@@ -764,6 +764,8 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     if (NC > 0 && !cons) return fail(GCDT_ERR_INVALID, "no constraints");
     long long nv = NP + 4LL * K, cap = 2LL * K + 4 * NP + 8;
     if (cap > (1LL << 30) || nv > (1LL << 29)) return fail(GCDT_ERR_INVALID, "batch too large");
+    // the per-instance location grid has 256 cells per instance
+    if (((long long)K << 8) > (1LL << 26)) return fail(GCDT_ERR_INVALID, "more than 262144 instances in one batch");
     hipStream_t st = (hipStream_t)stream;
     Ctx c(st);
     gcdt_batch_stats bs{};
@@ -777,43 +779,81 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     if (NP > 0) GCDT_HIP(hipMemcpyAsync(pts.p, points, 8 * NP, hipMemcpyDeviceToDevice, st));
     DBuf<int> vinst(NP > 0 ? NP : 1);
     batch_vinst(c, d_poff.p, K, NP, vinst.p);
-    // ---- insertion from every instance's two corner triangles
+    // ---- 1. sample meshes: one point per (instance, 16x16 cell), inserted from
+    //         every instance's two seed triangles, then made Delaunay
     Timer t1(c.st);
-    DBuf<int3> tA(cap), aA(cap);
-    DBuf<int> sA(cap), loc(nv), pend(NP > 0 ? NP : 1), pend2(NP > 0 ? NP : 1);
-    DBuf<unsigned long long> clA(cap), pkey(nv);
-    fill_i32(c, sA.p, -1, cap);
-    fill_u64(c, clA.p, 0, cap);
+    long long capA = 2LL * K + 4LL * std::min<long long>(NP, (long long)K << 8) + 8;
+    DBuf<int> sid(std::max<long long>(1, std::min<long long>(NP, (long long)K << 8)));
+    int S = batch_sample(c, pts.p, vinst.p, NP, K, sid.p);
+    DBuf<int3> tA(capA), aA(capA);
+    DBuf<int> sA(capA), loc(nv), pend(NP > 0 ? NP : 1), pend2(NP > 0 ? NP : 1);
+    DBuf<unsigned long long> clA(capA), pkey(nv);
+    fill_i32(c, sA.p, -1, capA);
+    fill_u64(c, clA.p, 0, capA);
     batch_init(c, pts.p, tA.p, aA.p, K, NP);
-    batch_locate(c, pts.p, vinst.p, NP, loc.p, pend.p);
+    batch_locate_ids(c, pts.p, vinst.p, sid.p, S, loc.p, pend.p);
+    sid.release();
     Ins s;
     s.m = Mesh{pts.p, tA.p, aA.p, sA.p, nullptr};
     s.loc = loc.p;
     s.claim = clA.p;
     s.pkey = pkey.p;
-    s.cap = (int)cap;
+    s.cap = (int)capA;
     s.err = c.dcnt.p + C_ERR;
     int ri = 0;
-    int nt = insert_rounds(c, s, pend, pend2, (int)NP, 2 * K, &ri);
+    int ntA = insert_rounds(c, s, pend, pend2, S, 2 * K, &ri);
+    long long capB = 2LL * K + 2LL * S + 4LL * (NP - S) + 8;
+    if (capB > (1LL << 31) - 1) throw Error(GCDT_ERR_INVALID, "batch too large");
+    DBuf<int3> tB(capB), aB(capB);
+    int Ts = filter_leaves(c, tA.p, aA.p, sA.p, ntA, tB.p, aB.p, capB);
+    tA.release();
+    aA.release();
+    sA.release();
     clA.release();
+    int fr = 0;
+    bs.flips += flip_rounds(c, Mesh{pts.p, tB.p, aB.p, nullptr, nullptr}, Ts, nullptr, 0, &fr);
+    // ---- 2. locate the other points through a per-instance bitmap of the
+    //         sample meshes, then insert them
+    int *bo = nullptr, *bt = nullptr, nbt = 0;
+    build_bitmap_batch(c, pts.p, vinst.p, NP, K, tB.p, Ts, &bo, &bt, &nbt);
+    DBuf<int> isv(NP > 0 ? NP : 1);
+    fill_i32(c, isv.p, 0, NP);
+    int np = locate_bitmap_batch(c, pts.p, vinst.p, NP, tB.p, Ts, bo, bt, loc.p, isv.p, pend.p);
+    getMemCacheRef().releaseAbtrtMem(bo, sizeof(int) * (size_t)(((long long)K << 8) + 1));
+    getMemCacheRef().releaseAbtrtMem(bt, sizeof(int) * (size_t)(nbt > 0 ? nbt : 1));
+    isv.release();
+    DBuf<int> sB(capB);
+    DBuf<unsigned long long> clB(capB);
+    fill_i32(c, sB.p, -1, capB);
+    fill_u64(c, clB.p, 0, capB);
+    Ins s2;
+    s2.m = Mesh{pts.p, tB.p, aB.p, sB.p, nullptr};
+    s2.loc = loc.p;
+    s2.claim = clB.p;
+    s2.pkey = pkey.p;
+    s2.cap = (int)capB;
+    s2.err = c.dcnt.p + C_ERR;
+    int ri2 = 0;
+    int nt = insert_rounds(c, s2, pend, pend2, np, Ts, &ri2);
+    clB.release();
     pkey.release();
     loc.release();
     pend.release();
     pend2.release();
     long long Tn = 2 * NP + 2LL * K;
     DBuf<int3> tri(Tn), adj(Tn);
-    int T = filter_leaves(c, tA.p, aA.p, sA.p, nt, tri.p, adj.p, Tn);
-    tA.release();
-    aA.release();
-    sA.release();
+    int T = filter_leaves(c, tB.p, aB.p, sB.p, nt, tri.p, adj.p, Tn);
+    tB.release();
+    aB.release();
+    sB.release();
     if (T != Tn) throw Error(GCDT_ERR_INTERNAL, "batch leaf count mismatch");
     bs.ms_insert = t1.ms();
-    bs.insert_rounds = ri;
+    bs.insert_rounds = ri + ri2;
     // ---- Delaunay flips of all instances together
     Timer t2(c.st);
-    int fr = 0;
-    bs.flips += flip_rounds(c, Mesh{pts.p, tri.p, adj.p, nullptr, nullptr}, T, nullptr, 0, &fr);
-    bs.flip_rounds = fr;
+    int fr2 = 0;
+    bs.flips += flip_rounds(c, Mesh{pts.p, tri.p, adj.p, nullptr, nullptr}, T, nullptr, 0, &fr2);
+    bs.flip_rounds = fr + fr2;
     bs.ms_flip = t2.ms();
     // ---- constraints
     Timer t3(c.st);

@@ -499,6 +499,31 @@ constexpr int LOG_BITMAP = 10;  // math_base.cuh:25 logBitmap
 constexpr int SAMPLE_CELLS = 1 << (2 * LOG_SAMPLE);
 constexpr int BITMAP_CELLS = 1 << (2 * LOG_BITMAP);
 
+// Cell grids of the point-location bitmap ("boostmap").  GridOne: one mesh,
+// 1024x1024 cells over the square (math_base.cuh:25 logBitmap).  GridInst:
+// a batch of instances that all span the square, each with its own 16x16
+// cells; a cell id is instance * 256 + local cell.
+struct GridOne {
+    static constexpr int LOG = LOG_BITMAP;
+    __device__ long long tri_base(const int3&) const { return 0; }
+    __device__ long long pt_base(int) const { return 0; }
+};
+
+struct GridInst {
+    static constexpr int LOG = 4;
+    const int* vinst;
+    long long NP;  // corners of instance i are NP + 4i .. NP + 4i + 3
+    __device__ int inst_of(int v) const { return v < NP ? vinst[v] : (int)((v - NP) >> 2); }
+    __device__ long long tri_base(const int3& t) const { return (long long)inst_of(t.x) << (2 * LOG); }
+    __device__ long long pt_base(int p) const { return (long long)vinst[p] << (2 * LOG); }
+};
+
+template <class G>
+__device__ inline long long pt_cell(const G& g, int p, int2 P)
+{
+    return g.pt_base(p) + ((long long)(P.x >> (30 - G::LOG)) << G::LOG) + (P.y >> (30 - G::LOG));
+}
+
 __global__ __launch_bounds__(kBlock) void k_sample_cells(const int2* pts, int n, int* cellmin)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -711,7 +736,8 @@ __global__ __launch_bounds__(kBlock) void k_mark_vertices(const int3* tri, int T
 
 // initTriangleBoost, rawMesh.cu:59-152: scan the triangles listed for the
 // point's bitmap cell; the first whose closed region holds the point wins.
-__global__ __launch_bounds__(kBlock) void k_locate_bitmap(const int2* pts, int n, const int* isv, const int3* tri,
+template <class G>
+__global__ __launch_bounds__(kBlock) void k_locate_bitmap(G g, const int2* pts, int n, const int* isv, const int3* tri,
                                                           const int* bm_off, const int* bm_tri, int* loc, int* pend,
                                                           int* npend, int* err)
 {
@@ -724,7 +750,7 @@ __global__ __launch_bounds__(kBlock) void k_locate_bitmap(const int2* pts, int n
                 loc[p] = -1;
             } else {
                 int2 P = pts[p];
-                int cell = ((P.x >> (30 - LOG_BITMAP)) << LOG_BITMAP) + (P.y >> (30 - LOG_BITMAP));
+                long long cell = pt_cell(g, p, P);
                 int r = -1, t = -1;
                 for (int j = bm_off[cell], e = bm_off[cell + 1]; j < e; j++) {
                     t = bm_tri[j];
@@ -746,20 +772,33 @@ __global__ __launch_bounds__(kBlock) void k_locate_bitmap(const int2* pts, int n
     }
 }
 
-int locate_bitmap(Ctx& c, const int2* pts, int n, const int3* tri, int T, const int* bm_off, const int* bm_tri,
-                  int* loc, int* isv, int* pend)
+template <class G>
+static int locate_bitmap_g(Ctx& c, const G& g, const int2* pts, int n, const int3* tri, int T, const int* bm_off,
+                           const int* bm_tri, int* loc, int* isv, int* pend)
 {
     k_mark_vertices<<<grid_for(T), kBlock, 0, c.st>>>(tri, T, n, isv);
     GCDT_LAUNCH_CHECK();
     c.set(C_NPEND, 0);
     if (n > 0) {
         ProfScope ps("locate_bitmap", c.st, n);
-        k_locate_bitmap<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, isv, tri, bm_off, bm_tri, loc, pend,
-                                                          c.dcnt.p + C_NPEND, c.dcnt.p + C_ERR);
+        k_locate_bitmap<G><<<grid_for(n), kBlock, 0, c.st>>>(g, pts, n, isv, tri, bm_off, bm_tri, loc, pend,
+                                                             c.dcnt.p + C_NPEND, c.dcnt.p + C_ERR);
         GCDT_LAUNCH_CHECK();
     }
     c.check_err();
     return c.get(C_NPEND);
+}
+
+int locate_bitmap(Ctx& c, const int2* pts, int n, const int3* tri, int T, const int* bm_off, const int* bm_tri,
+                  int* loc, int* isv, int* pend)
+{
+    return locate_bitmap_g(c, GridOne{}, pts, n, tri, T, bm_off, bm_tri, loc, isv, pend);
+}
+
+int locate_bitmap_batch(Ctx& c, const int2* pts, const int* vinst, long long NP, const int3* tri, int T,
+                        const int* bm_off, const int* bm_tri, int* loc, int* isv, int* pend)
+{
+    return locate_bitmap_g(c, GridInst{vinst, NP}, pts, (int)NP, tri, T, bm_off, bm_tri, loc, isv, pend);
 }
 
 // ================================================================== insertion rounds
@@ -1487,6 +1526,66 @@ void batch_vinst(Ctx& c, const long long* off, int K, long long NP, int* vinst)
     GCDT_LAUNCH_CHECK();
 }
 
+// Batch sample: the smallest point id of each (instance, 16x16 cell).
+__global__ __launch_bounds__(kBlock) void k_batch_sample_cells(const int2* pts, const int* vinst, long long NP,
+                                                               int* cellmin)
+{
+    GridInst g{vinst, NP};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x)
+        atomicMin(&cellmin[pt_cell(g, (int)i, pts[i])], (int)i);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_sample_flags(const int* cellmin, long long ncells, int* flags)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long long)gridDim.x * blockDim.x)
+        flags[i] = cellmin[i] != INT_MAX;
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_sample_emit(const int* cellmin, const int* offs, long long ncells,
+                                                              int* ids)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long long)gridDim.x * blockDim.x)
+        if (cellmin[i] != INT_MAX) ids[offs[i]] = cellmin[i];
+}
+
+int batch_sample(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, int* ids)
+{
+    long long ncells = (long long)K << (2 * GridInst::LOG);
+    DBuf<int> cellmin((size_t)ncells), offs((size_t)ncells + 1);
+    fill_i32(c, cellmin.p, INT_MAX, ncells);
+    if (NP > 0) {
+        k_batch_sample_cells<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, cellmin.p);
+        GCDT_LAUNCH_CHECK();
+    }
+    k_batch_sample_flags<<<grid_for(ncells), kBlock, 0, c.st>>>(cellmin.p, ncells, offs.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl(c, offs.p, offs.p, ncells);
+    k_batch_sample_emit<<<grid_for(ncells), kBlock, 0, c.st>>>(cellmin.p, offs.p, ncells, ids);
+    GCDT_LAUNCH_CHECK();
+    return c.get(C_TOTAL);
+}
+
+// Listed points located in their instance's two seed triangles (the side of
+// the anti-diagonal, as k_batch_locate).
+__global__ __launch_bounds__(kBlock) void k_batch_locate_ids(const int2* pts, const int* vinst, const int* ids, int n,
+                                                             int* loc, int* pend)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        int p = ids[i];
+        int t0 = 2 * vinst[p];
+        int o = orient2d(make_int2(GRID_R, 0), make_int2(0, GRID_R), pts[p]);
+        loc[p] = o > 0 ? t0 * 4 + 3 : (o < 0 ? (t0 + 1) * 4 + 3 : t0 * 4 + 0);
+        pend[i] = p;
+    }
+}
+
+void batch_locate_ids(Ctx& c, const int2* pts, const int* vinst, const int* ids, int n, int* loc, int* pend)
+{
+    if (n <= 0) return;
+    k_batch_locate_ids<<<grid_for(n), kBlock, 0, c.st>>>(pts, vinst, ids, n, loc, pend);
+    GCDT_LAUNCH_CHECK();
+}
+
 void batch_init(Ctx& c, int2* pts, int3* tri, int3* adj, int K, long long NP)
 {
     k_batch_init<<<grid_for(K), kBlock, 0, c.st>>>(pts, tri, adj, K, NP);
@@ -1638,9 +1737,10 @@ int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int
 
 // ================================================================== bitmap
 
+template <int LOG>
 __device__ inline void tri_cells(int2 A, int2 B, int2 C, int& x0, int& y0, int& x1, int& y1)
 {
-    constexpr int sh = 30 - LOG_BITMAP;
+    constexpr int sh = 30 - LOG;
     x0 = min(A.x, min(B.x, C.x)) >> sh;
     x1 = max(A.x, max(B.x, C.x)) >> sh;
     y0 = min(A.y, min(B.y, C.y)) >> sh;
@@ -1667,12 +1767,13 @@ __device__ inline bool tri_meets_box(int2 A, int2 B, int2 C, int bx0, int by0, i
     return true;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bm_count(const int2* pts, const int3* tri, int T, int* cnt)
+template <class G>
+__global__ __launch_bounds__(kBlock) void k_bm_count(G g, const int2* pts, const int3* tri, int T, int* cnt)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
         int3 t = tri[i];
         int x0, y0, x1, y1;
-        tri_cells(pts[t.x], pts[t.y], pts[t.z], x0, y0, x1, y1);
+        tri_cells<G::LOG>(pts[t.x], pts[t.y], pts[t.z], x0, y0, x1, y1);
         cnt[i] = (x1 - x0 + 1) * (y1 - y0 + 1);
     }
 }
@@ -1688,7 +1789,8 @@ __device__ inline int upper_bound_i(const int* a, int n, long long x)
     return lo;
 }
 
-__global__ __launch_bounds__(kBlock) void k_bm_test(const int2* pts, const int3* tri, const int* off, int T,
+template <class G>
+__global__ __launch_bounds__(kBlock) void k_bm_test(G g, const int2* pts, const int3* tri, const int* off, int T,
                                                     long long npairs, int* cellcnt, int* ord, int* cellof)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += (long long)gridDim.x * blockDim.x) {
@@ -1697,12 +1799,12 @@ __global__ __launch_bounds__(kBlock) void k_bm_test(const int2* pts, const int3*
         int3 tv = tri[t];
         int2 A = pts[tv.x], B = pts[tv.y], C = pts[tv.z];
         int x0, y0, x1, y1;
-        tri_cells(A, B, C, x0, y0, x1, y1);
+        tri_cells<G::LOG>(A, B, C, x0, y0, x1, y1);
         int h = y1 - y0 + 1;
         int cx = x0 + local / h, cy = y0 + local % h;
-        constexpr int sh = 30 - LOG_BITMAP;
+        constexpr int sh = 30 - G::LOG;
         bool hit = tri_meets_box(A, B, C, cx << sh, cy << sh, ((cx + 1) << sh) - 1, ((cy + 1) << sh) - 1);
-        int cell = (cx << LOG_BITMAP) + cy;
+        int cell = (int)(g.tri_base(tv) + ((long long)cx << G::LOG) + cy);
         cellof[i] = cell;
         ord[i] = hit ? atomicAdd(&cellcnt[cell], 1) : -1;
     }
@@ -1719,21 +1821,24 @@ __global__ __launch_bounds__(kBlock) void k_bm_fill(const int* off, int T, long 
     }
 }
 
-void build_bitmap(Ctx& c, const int2* pts, const int3* tri, int T, int** offsets, int** tris, int* ntris)
+template <class G>
+static void build_bitmap_g(Ctx& c, const G& g, long long ncells, const int2* pts, const int3* tri, int T,
+                           int** offsets, int** tris, int* ntris)
 {
     DBuf<int> off((size_t)T + 1);
-    k_bm_count<<<grid_for(T), kBlock, 0, c.st>>>(pts, tri, T, off.p);
+    k_bm_count<G><<<grid_for(T), kBlock, 0, c.st>>>(g, pts, tri, T, off.p);
     GCDT_LAUNCH_CHECK();
     scan_excl_total(c, off.p, off.p, T);
     long long npairs = c.get(C_TOTAL);
     DBuf<int> ord(npairs), cellof(npairs);
-    DBuf<int> cell_off(BITMAP_CELLS + 1);
-    fill_i32(c, cell_off.p, 0, BITMAP_CELLS + 1);
+    DBuf<int> cell_off((size_t)ncells + 1);
+    fill_i32(c, cell_off.p, 0, ncells + 1);
     if (npairs > 0) {
-        k_bm_test<<<grid_for(npairs), kBlock, 0, c.st>>>(pts, tri, off.p, T, npairs, cell_off.p, ord.p, cellof.p);
+        k_bm_test<G><<<grid_for(npairs), kBlock, 0, c.st>>>(g, pts, tri, off.p, T, npairs, cell_off.p, ord.p,
+                                                            cellof.p);
         GCDT_LAUNCH_CHECK();
     }
-    scan_excl_total(c, cell_off.p, cell_off.p, BITMAP_CELLS);
+    scan_excl_total(c, cell_off.p, cell_off.p, ncells);
     int total = c.get(C_TOTAL);
     DBuf<int> bm(total > 0 ? total : 1);
     if (npairs > 0) {
@@ -1741,10 +1846,21 @@ void build_bitmap(Ctx& c, const int2* pts, const int3* tri, int T, int** offsets
         GCDT_LAUNCH_CHECK();
     }
     // hand the buffers to the caller; they are released with the sizes
-    // (2^20+1) ints and `total` ints
+    // (ncells + 1) ints and `total` ints
     *offsets = cell_off.detach();
     *tris = bm.detach();
     *ntris = total;
+}
+
+void build_bitmap(Ctx& c, const int2* pts, const int3* tri, int T, int** offsets, int** tris, int* ntris)
+{
+    build_bitmap_g(c, GridOne{}, BITMAP_CELLS, pts, tri, T, offsets, tris, ntris);
+}
+
+void build_bitmap_batch(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri, int T,
+                        int** offsets, int** tris, int* ntris)
+{
+    build_bitmap_g(c, GridInst{vinst, NP}, (long long)K << (2 * GridInst::LOG), pts, tri, T, offsets, tris, ntris);
 }
 
 // ================================================================== adjacency from a triangle list

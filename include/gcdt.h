@@ -231,7 +231,10 @@ typedef struct {
  * NULL when there are no constraints).  Instance i's 2*N_i+2 triangles go to
  * tris_out[sum_{j<i} (2*N_j+2) ...] with local vertex ids (corners
  * N_i..N_i+3).  Every round of every phase runs for all instances at once,
- * so a batch costs about as many rounds as its largest instance. */
+ * so a batch costs about as many rounds as its largest instance.  Like
+ * gcdt_solve, each instance first triangulates a sample (one point per cell
+ * of its own 16x16 grid), makes it Delaunay and locates the other points
+ * through a per-instance bitmap of it.  K <= 262144. */
 int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int K, const gcdt_int2* cons,
                      const int64_t* cons_offsets, gcdt_int3* tris_out, gcdt_batch_stats* stats, void* stream);
 
