@@ -775,16 +775,19 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     GCDT_HIP(hipMemcpyAsync(d_coff.p, coff.data(), 8 * (K + 1), hipMemcpyHostToDevice, st));
     GCDT_HIP(hipMemcpyAsync(d_toff.p, toff.data(), 8 * (K + 1), hipMemcpyHostToDevice, st));
     check_points(c, (const int2*)points, (int)NP);
-    DBuf<int2> pts(nv);
-    if (NP > 0) GCDT_HIP(hipMemcpyAsync(pts.p, points, 8 * NP, hipMemcpyDeviceToDevice, st));
     DBuf<int> vinst(NP > 0 ? NP : 1);
     batch_vinst(c, d_poff.p, K, NP, vinst.p);
+    // points in (instance, cell) order for locality; ids are mapped back at the end
+    DBuf<int2> pts(nv);
+    DBuf<int> perm(NP > 0 ? NP : 1), inv(NP > 0 ? NP : 1), cell_end(((size_t)K << 8) + 1);
+    batch_order(c, (const int2*)points, vinst.p, NP, K, pts.p, perm.p, inv.p, cell_end.p);
     // ---- 1. sample meshes: one point per (instance, 16x16 cell), inserted from
     //         every instance's two seed triangles, then made Delaunay
     Timer t1(c.st);
     long long capA = 2LL * K + 4LL * std::min<long long>(NP, (long long)K << 8) + 8;
     DBuf<int> sid(std::max<long long>(1, std::min<long long>(NP, (long long)K << 8)));
-    int S = batch_sample(c, pts.p, vinst.p, NP, K, sid.p);
+    int S = batch_sample_sorted(c, cell_end.p, K, sid.p);
+    cell_end.release();
     DBuf<int3> tA(capA), aA(capA);
     DBuf<int> sA(capA), loc(nv), pend(NP > 0 ? NP : 1), pend2(NP > 0 ? NP : 1);
     DBuf<unsigned long long> clA(capA), pkey(nv);
@@ -860,6 +863,7 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     if (NC > 0) {
         DBuf<int2> cg(NC);
         batch_cons(c, (const int2*)cons, d_coff.p, d_poff.p, K, NC, cg.p);
+        relabel_pairs(c, cg.p, (int)NC, inv.p, (int)NP, cg.p);
         DBuf<int> cflag(T), touched(T), tl(T);
         fill_i32(c, cflag.p, 0, T);
         fill_i32(c, touched.p, 0, T);
@@ -875,7 +879,7 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
         }
     }
     bs.ms_constraints = t3.ms();
-    batch_out(c, tri.p, T, vinst.p, d_poff.p, d_toff.p, NP, K, (int3*)tris_out);
+    batch_out(c, tri.p, T, vinst.p, perm.p, d_poff.p, d_toff.p, NP, K, (int3*)tris_out);
     bs.ms_total = total.ms();
     bs.num_instances = K;
     bs.num_points = NP;

@@ -1470,16 +1470,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(int2* pts, int3* tri, int
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_locate(const int2* pts, const int* vinst, long long NP, int* loc, int* pend)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
-        int t0 = 2 * vinst[i];
-        int o = orient2d(make_int2(GRID_R, 0), make_int2(0, GRID_R), pts[i]);
-        loc[i] = o > 0 ? t0 * 4 + 3 : (o < 0 ? (t0 + 1) * 4 + 3 : t0 * 4 + 0);
-        pend[i] = (int)i;
-    }
-}
-
 // constraint vertex ids: instance-local -> global; ids outside the instance
 // become -1 (rejected by the constraint checks)
 __global__ __launch_bounds__(kBlock) void k_batch_cons(const int2* in, const long long* coff, const long long* poff, int K,
@@ -1501,15 +1491,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_cons(const int2* in, const lon
 }
 
 // scatter the global mesh back into per-instance lists with local vertex ids
-__global__ __launch_bounds__(kBlock) void k_batch_out(const int3* tri, int T, const int* vinst, const long long* poff,
-                                                      const long long* toff, long long NP, int* cursor, int3* out,
-                                                      int* err)
+__global__ __launch_bounds__(kBlock) void k_batch_out(const int3* tri, int T, const int* vinst, const int* perm,
+                                                      const long long* poff, const long long* toff, long long NP,
+                                                      int* cursor, int3* out, int* err)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
         int3 t = tri[i];
         int inst = t.x < NP ? vinst[t.x] : (int)((t.x - NP) >> 2);
         long long base = poff[inst], n = poff[inst + 1] - poff[inst];
-        auto local = [&](int v) -> int { return v < NP ? (int)(v - base) : (int)(n + ((v - NP) & 3)); };
+        auto local = [&](int v) -> int {
+            return v < NP ? (int)((perm ? perm[v] : v) - base) : (int)(n + ((v - NP) & 3));
+        };
         long long pos = toff[inst] + atomicAdd(&cursor[inst], 1);
         if (pos >= toff[inst + 1]) {
             atomicOr(err, MERR_INTERNAL_COUNT);
@@ -1526,47 +1518,71 @@ void batch_vinst(Ctx& c, const long long* off, int K, long long NP, int* vinst)
     GCDT_LAUNCH_CHECK();
 }
 
-// Batch sample: the smallest point id of each (instance, 16x16 cell).
-__global__ __launch_bounds__(kBlock) void k_batch_sample_cells(const int2* pts, const int* vinst, long long NP,
-                                                               int* cellmin)
+// Batch points counting-sorted by (instance, 16x16 cell): instance ranges are
+// kept (the key is instance-major), neighbours in the plane become neighbours
+// in memory, and every cell's range start is a sample point.
+__global__ __launch_bounds__(kBlock) void k_batch_cell_count(const int2* pts, const int* vinst, long long NP, int* cnt)
 {
     GridInst g{vinst, NP};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x)
-        atomicMin(&cellmin[pt_cell(g, (int)i, pts[i])], (int)i);
+        atomicAdd(&cnt[pt_cell(g, (int)i, pts[i])], 1);
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_sample_flags(const int* cellmin, long long ncells, int* flags)
+__global__ __launch_bounds__(kBlock) void k_batch_cell_scatter(const int2* pts, const int* vinst, long long NP,
+                                                               int* cursor, int2* out, int* perm, int* inv)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long long)gridDim.x * blockDim.x)
-        flags[i] = cellmin[i] != INT_MAX;
+    GridInst g{vinst, NP};
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
+        int2 p = pts[i];
+        int slot = atomicAdd(&cursor[pt_cell(g, (int)i, p)], 1);
+        out[slot] = p;
+        perm[slot] = (int)i;
+        inv[i] = slot;
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_sample_emit(const int* cellmin, const int* offs, long long ncells,
-                                                              int* ids)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long long)gridDim.x * blockDim.x)
-        if (cellmin[i] != INT_MAX) ids[offs[i]] = cellmin[i];
-}
-
-int batch_sample(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, int* ids)
+void batch_order(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, int2* out, int* perm, int* inv,
+                 int* cell_end)
 {
     long long ncells = (long long)K << (2 * GridInst::LOG);
-    DBuf<int> cellmin((size_t)ncells), offs((size_t)ncells + 1);
-    fill_i32(c, cellmin.p, INT_MAX, ncells);
-    if (NP > 0) {
-        k_batch_sample_cells<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, cellmin.p);
-        GCDT_LAUNCH_CHECK();
+    fill_i32(c, cell_end, 0, ncells + 1);
+    if (NP <= 0) return;
+    k_batch_cell_count<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, cell_end);
+    GCDT_LAUNCH_CHECK();
+    scan_excl(c, cell_end, cell_end, ncells);
+    // the scatter advances each cell's cursor to the cell's end
+    k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, cell_end, out, perm, inv);
+    GCDT_LAUNCH_CHECK();
+}
+
+__global__ __launch_bounds__(kBlock) void k_range_flags(const int* cell_end, long long ncells, int* flags)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long long)gridDim.x * blockDim.x)
+        flags[i] = cell_end[i] > (i ? cell_end[i - 1] : 0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_range_emit(const int* cell_end, const int* offs, long long ncells, int* ids)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long long)gridDim.x * blockDim.x) {
+        int a = i ? cell_end[i - 1] : 0;
+        if (cell_end[i] > a) ids[offs[i]] = a;
     }
-    k_batch_sample_flags<<<grid_for(ncells), kBlock, 0, c.st>>>(cellmin.p, ncells, offs.p);
+}
+
+int batch_sample_sorted(Ctx& c, const int* cell_end, int K, int* ids)
+{
+    long long ncells = (long long)K << (2 * GridInst::LOG);
+    DBuf<int> offs((size_t)ncells + 1);
+    k_range_flags<<<grid_for(ncells), kBlock, 0, c.st>>>(cell_end, ncells, offs.p);
     GCDT_LAUNCH_CHECK();
     scan_excl(c, offs.p, offs.p, ncells);
-    k_batch_sample_emit<<<grid_for(ncells), kBlock, 0, c.st>>>(cellmin.p, offs.p, ncells, ids);
+    k_range_emit<<<grid_for(ncells), kBlock, 0, c.st>>>(cell_end, offs.p, ncells, ids);
     GCDT_LAUNCH_CHECK();
     return c.get(C_TOTAL);
 }
 
 // Listed points located in their instance's two seed triangles (the side of
-// the anti-diagonal, as k_batch_locate).
+// the anti-diagonal).
 __global__ __launch_bounds__(kBlock) void k_batch_locate_ids(const int2* pts, const int* vinst, const int* ids, int n,
                                                              int* loc, int* pend)
 {
@@ -1592,13 +1608,6 @@ void batch_init(Ctx& c, int2* pts, int3* tri, int3* adj, int K, long long NP)
     GCDT_LAUNCH_CHECK();
 }
 
-void batch_locate(Ctx& c, const int2* pts, const int* vinst, long long NP, int* loc, int* pend)
-{
-    if (NP <= 0) return;
-    k_batch_locate<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, loc, pend);
-    GCDT_LAUNCH_CHECK();
-}
-
 void batch_cons(Ctx& c, const int2* in, const long long* coff, const long long* poff, int K, long long NC, int2* out)
 {
     if (NC <= 0) return;
@@ -1606,12 +1615,13 @@ void batch_cons(Ctx& c, const int2* in, const long long* coff, const long long* 
     GCDT_LAUNCH_CHECK();
 }
 
-void batch_out(Ctx& c, const int3* tri, int T, const int* vinst, const long long* poff, const long long* toff,
-               long long NP, int K, int3* out)
+void batch_out(Ctx& c, const int3* tri, int T, const int* vinst, const int* perm, const long long* poff,
+               const long long* toff, long long NP, int K, int3* out)
 {
     DBuf<int> cursor(K);
     fill_i32(c, cursor.p, 0, K);
-    k_batch_out<<<grid_for(T), kBlock, 0, c.st>>>(tri, T, vinst, poff, toff, NP, cursor.p, out, c.dcnt.p + C_ERR);
+    k_batch_out<<<grid_for(T), kBlock, 0, c.st>>>(tri, T, vinst, perm, poff, toff, NP, cursor.p, out,
+                                                  c.dcnt.p + C_ERR);
     GCDT_LAUNCH_CHECK();
     c.check_err();
 }

@@ -209,9 +209,12 @@ void unmap_tris(Ctx& c, int3* tri, int T, const int* perm, int n);
 // constraint ids -> global, and the per-instance output scatter
 void batch_vinst(Ctx& c, const long long* off, int K, long long NP, int* vinst);
 void batch_init(Ctx& c, int2* pts, int3* tri, int3* adj, int K, long long NP);
-void batch_locate(Ctx& c, const int2* pts, const int* vinst, long long NP, int* loc, int* pend);
-// one sample point per (instance, 16x16 cell): the smallest id; returns the count
-int batch_sample(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, int* ids);
+// points counting-sorted by (instance, 16x16 cell); cell_end[K*256 + 1]
+// receives each cell's end offset
+void batch_order(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, int2* out, int* perm, int* inv,
+                 int* cell_end);
+// the first point of every non-empty cell of batch_order's output
+int batch_sample_sorted(Ctx& c, const int* cell_end, int K, int* ids);
 // listed points into their instance's seed triangles
 void batch_locate_ids(Ctx& c, const int2* pts, const int* vinst, const int* ids, int n, int* loc, int* pend);
 // bitmap of a batch's sample meshes: 16x16 cells per instance; offsets has
@@ -221,8 +224,10 @@ void build_bitmap_batch(Ctx& c, const int2* pts, const int* vinst, long long NP,
 int locate_bitmap_batch(Ctx& c, const int2* pts, const int* vinst, long long NP, const int3* tri, int T,
                         const int* bm_off, const int* bm_tri, int* loc, int* isv, int* pend);
 void batch_cons(Ctx& c, const int2* in, const long long* coff, const long long* poff, int K, long long NC, int2* out);
-void batch_out(Ctx& c, const int3* tri, int T, const int* vinst, const long long* poff, const long long* toff,
-               long long NP, int K, int3* out);
+// per-instance triangle lists with local vertex ids; perm (nullable) maps the
+// batch's internal point order back to the caller's
+void batch_out(Ctx& c, const int3* tri, int T, const int* vinst, const int* perm, const long long* poff,
+               const long long* toff, long long NP, int K, int3* out);
 // indices i with flags[i] != 0 (flags are 0/1), ascending; returns the count
 int flags_to_list(Ctx& c, const int* flags, int n, int* list);
 // leaves of a DAG -> compact arrays (adjacency remapped); returns count
