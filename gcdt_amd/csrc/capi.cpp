@@ -131,6 +131,12 @@ int gcdt_pool_set_budget(size_t bytes)
     return GCDT_OK;
 }
 
+int gcdt_pool_trim(void)
+{
+    getMemCacheRef().releaseCache();
+    return GCDT_OK;
+}
+
 int gcdt_memcpy_h2d(void* dst, const void* src, size_t bytes)
 {
     API_BEGIN

@@ -17,10 +17,30 @@ static int classOf(size_t size)
     return p == kSizes + kClasses ? -1 : (int)(p - kSizes);
 }
 
+// Requests above the largest class are rounded up to 64 MiB and cached too:
+// freeing multi-GB buffers on every solve would synchronise the device in
+// the middle of the pipelined rounds and pay the mapping cost again.
+static const size_t kLargeStep = size_t(64) << 20;
+static size_t largeBytes(size_t size) { return (size + kLargeStep - 1) / kLargeStep * kLargeStep; }
+
 size_t MemCache::classBytes(size_t size)
 {
     int c = classOf(size);
-    return c < 0 ? size : kSizes[c];
+    return c < 0 ? largeBytes(size) : kSizes[c];
+}
+
+// Returns every cached block to the device (the caller holds mu_).
+void MemCache::freeCachedLocked()
+{
+    for (auto& kv : large_)
+        for (void* p : kv.second) (void)hipFree(p);
+    large_.clear();
+    for (auto& s : stkInVec)
+        while (!s.empty()) {
+            (void)hipFree(s.top());
+            s.pop();
+        }
+    cached_ = 0;
 }
 
 void MemCache::ensureClasses()
@@ -37,13 +57,23 @@ void* MemCache::getMemByAbtrtSize(size_t size)
     ensureClasses();
     void* mem = nullptr;
     int c = classOf(size);
-    size_t bytes = c < 0 ? size : kSizes[c];
+    size_t bytes = c < 0 ? largeBytes(size) : kSizes[c];
     if (c >= 0 && !stkInVec[c].empty()) {
         mem = stkInVec[c].top();
         stkInVec[c].pop();
         cached_ -= bytes;
-    } else {
-        if (hipMalloc(&mem, bytes == 0 ? 1 : bytes) != hipSuccess) return nullptr;
+    } else if (c < 0 && !large_[bytes].empty()) {
+        mem = large_[bytes].back();
+        large_[bytes].pop_back();
+        cached_ -= bytes;
+    } else if (hipMalloc(&mem, bytes == 0 ? 1 : bytes) != hipSuccess) {
+        // out of device memory: give the cache back and try once more
+        (void)hipGetLastError();
+        freeCachedLocked();
+        if (hipMalloc(&mem, bytes == 0 ? 1 : bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
     }
     inUse_ += bytes;
     peak_ = std::max(peak_, inUse_);
@@ -56,14 +86,11 @@ void MemCache::releaseAbtrtMem(void* ptr, size_t size)
     std::lock_guard<std::mutex> g(mu_);
     ensureClasses();
     int c = classOf(size);
-    size_t bytes = c < 0 ? size : kSizes[c];
+    size_t bytes = c < 0 ? largeBytes(size) : kSizes[c];
     inUse_ -= std::min(inUse_, bytes);
-    if (c < 0) {
-        (void)hipFree(ptr);
-    } else {
-        stkInVec[c].push(ptr);
-        cached_ += bytes;
-    }
+    if (c < 0) large_[bytes].push_back(ptr);
+    else stkInVec[c].push(ptr);
+    cached_ += bytes;
 }
 
 void* MemCache::getMemBySize(size_t size)
@@ -104,12 +131,7 @@ void MemCache::releaseCache()
     for (auto& kv : unordered_mp)
         for (void* p : kv.second) (void)hipFree(p);
     unordered_mp.clear();
-    for (auto& s : stkInVec)
-        while (!s.empty()) {
-            (void)hipFree(s.top());
-            s.pop();
-        }
-    cached_ = 0;
+    freeCachedLocked();
     if (m_pinnedPage) (void)hipHostFree(m_pinnedPage);
     m_pinnedPage = nullptr;
 }

@@ -1675,15 +1675,22 @@ __global__ __launch_bounds__(kBlock) void k_leaf_scatter(const int3* tri, const 
 // Leaves in Morton order of their centroid's 1024^2 cell (the reference's
 // reorderFaces, kernel.cu:1886-1964, done as a counting sort fused with the
 // compaction): neighbouring triangles end up near each other in memory.
-__global__ __launch_bounds__(kBlock) void k_leaf_cells(const int2* pts, const int3* tri, const int* sons, int T, int* cnt,
-                                                       int* cellof)
+// Sort key of a leaf for the compacted mesh order: the Morton cell of its
+// centroid on the 1024x1024 grid over the square.
+struct LeafKeyOne {
+    __device__ int key(const int3&, int2 g) const { return (int)morton_cell(g); }
+};
+
+template <class KEY>
+__global__ __launch_bounds__(kBlock) void k_leaf_cells(KEY kf, const int2* pts, const int3* tri, const int* sons, int T,
+                                                       int* cnt, int* cellof)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
         if (sons[i] != -1) continue;
         int3 t = tri[i];
         int2 a = pts[t.x], b = pts[t.y], c = pts[t.z];
         int2 g = make_int2((int)(((long long)a.x + b.x + c.x) / 3), (int)(((long long)a.y + b.y + c.y) / 3));
-        int cell = (int)morton_cell(g);
+        int cell = kf.key(t, g);
         cellof[i] = cell;
         atomicAdd(&cnt[cell], 1);
     }
@@ -1714,14 +1721,15 @@ __global__ __launch_bounds__(kBlock) void k_leaf_move(const int3* tri, const int
     }
 }
 
-int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2,
-                         int3* adj2, long long cap)
+template <class KEY>
+static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const int2* pts, const int3* tri,
+                                const int3* adj, const int* sons, int T, int3* tri2, int3* adj2, long long cap)
 {
-    DBuf<int> cnt(BITMAP_CELLS + 1), cellof(T > 0 ? T : 1), dest(T > 0 ? T : 1);
-    fill_i32(c, cnt.p, 0, BITMAP_CELLS + 1);
-    k_leaf_cells<<<grid_for(T), kBlock, 0, c.st>>>(pts, tri, sons, T, cnt.p, cellof.p);
+    DBuf<int> cnt((size_t)ncells + 1), cellof(T > 0 ? T : 1), dest(T > 0 ? T : 1);
+    fill_i32(c, cnt.p, 0, ncells + 1);
+    k_leaf_cells<KEY><<<grid_for(T), kBlock, 0, c.st>>>(kf, pts, tri, sons, T, cnt.p, cellof.p);
     GCDT_LAUNCH_CHECK();
-    scan_excl(c, cnt.p, cnt.p, BITMAP_CELLS);
+    scan_excl(c, cnt.p, cnt.p, ncells);
     int n = c.get(C_TOTAL);
     k_leaf_dest<<<grid_for(T), kBlock, 0, c.st>>>(sons, cellof.p, T, cnt.p, dest.p);
     GCDT_LAUNCH_CHECK();
@@ -1729,6 +1737,12 @@ int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* a
     GCDT_LAUNCH_CHECK();
     c.check_err();
     return n;
+}
+
+int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2,
+                         int3* adj2, long long cap)
+{
+    return filter_leaves_sorted(c, LeafKeyOne{}, BITMAP_CELLS, pts, tri, adj, sons, T, tri2, adj2, cap);
 }
 
 int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2, int3* adj2,

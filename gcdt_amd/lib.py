@@ -71,6 +71,7 @@ SIGNATURES = {
     "gcdt_pool_init": ([], i32),
     "gcdt_pool_stats": ([ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)], i32),
     "gcdt_pool_set_budget": ([sz], i32),
+    "gcdt_pool_trim": ([], i32),
     "gcdt_memcpy_h2d": ([vp, vp, sz], i32),
     "gcdt_memcpy_d2h": ([vp, vp, sz], i32),
     "gcdt_memcpy_d2d": ([vp, vp, sz], i32),

@@ -62,6 +62,8 @@ int gcdt_release(void* ptr, size_t bytes);
 int gcdt_pool_init(void);                 /* MemCache::initAbtrtMem */
 int gcdt_pool_stats(size_t* in_use, size_t* cached, size_t* peak);
 int gcdt_pool_set_budget(size_t bytes);
+/* Returns every cached (free) pool block to the device; blocks in use stay. */
+int gcdt_pool_trim(void);
 int gcdt_memcpy_h2d(void* dst, const void* host_src, size_t bytes);
 int gcdt_memcpy_d2h(void* host_dst, const void* src, size_t bytes);
 int gcdt_memcpy_d2d(void* dst, const void* src, size_t bytes);
