@@ -399,7 +399,12 @@ def bench_degenerate(args):
     dc = DeviceBuffer.from_array(cons)
     dt = DeviceBuffer(12 * (2 * n + 2))
     da = DeviceBuffer(12 * (2 * n + 2))
+    tuned = None
     try:
+        if not args.no_tune:
+            from gcdt_amd import tune
+            tuned = tune.load_or_tune("degenerate", lambda: solve_device(dp, n, dc, c, dt, da), repeats=2)
+            print(f"# tunables (degenerate): {tuned}", file=sys.stderr, flush=True)
         solve_device(dp, n, dc, c, dt, da)
         T, st = solve_device(dp, n, dc, c, dt, da)
         tris = dt.to_array(np.int32, (T, 3))
@@ -414,7 +419,7 @@ def bench_degenerate(args):
             "segments_after_split": st["segments"], "flips": st["flips"],
             "rounds": {"insert": st["insert_rounds"], "flip": st["flip_rounds"], "constraints": st["constraint_rounds"]},
             "valid": v["valid"], "not_delaunay": v["not_delaunay"], "missing_constraints": v["missing_constraints"],
-            "cocircular_edges": v["cocircular"]}
+            "cocircular_edges": v["cocircular"], "tunables": tuned["choice"] if tuned else "defaults"}
 
 
 def bench_serving(args):

@@ -43,3 +43,15 @@ def test_file_to_canonical_mesh(tmp_path):
     if v["unique"]:
         assert np.array_equal(out, O.canonical(ref))
     gio.write_mesh(str(tmp_path / "mesh.txt"), g, out)
+
+
+def test_reference_four_point_case_through_the_input_path():
+    """kernel.cu:49-52 points and the chain of kernel.cu:5348-5351, mapped on
+    the GPU by gcdt_trans_points (kernel.cu:171-188) and solved: the result is
+    the brute-force CDT."""
+    f = np.array([[0.75, 0.5], [0.75, 0.2], [0.25, 0.2], [0.5, 0.75]])
+    g = gio.trans_points(f)
+    assert np.array_equal(g, IO.trans_points(f))
+    cons = np.array([[0, 1], [1, 2], [2, 3]], np.int32)
+    tris, _ = solve(g, cons)
+    assert np.array_equal(gio.canonical(tris), O.brute_force_cdt(g, cons))
