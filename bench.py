@@ -75,8 +75,8 @@ def _prof(lib, name):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--points", type=int, default=10_000_000)
     ap.add_argument("--circles", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -403,7 +403,7 @@ def bench_degenerate(args):
     try:
         if not args.no_tune:
             from gcdt_amd import tune
-            tuned = tune.load_or_tune("degenerate", lambda: solve_device(dp, n, dc, c, dt, da), repeats=2)
+            tuned = tune.load_or_tune("degenerate", lambda: solve_device(dp, n, dc, c, dt, da), repeats=3)
             print(f"# tunables (degenerate): {tuned}", file=sys.stderr, flush=True)
         solve_device(dp, n, dc, c, dt, da)
         T, st = solve_device(dp, n, dc, c, dt, da)

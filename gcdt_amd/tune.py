@@ -7,7 +7,11 @@ Two knobs decide how the round loops map onto the GPU:
 * ``grid_cap`` -- workgroups of a grid-stride launch;
 * ``flip_items`` / ``ins_items`` -- list items per thread of the kernels that
   append to a list once per block (more items amortise the append, fewer keep
-  more waves in flight).
+  more waves in flight);
+* ``pivot`` -- which pending point of a triangle an insertion round inserts
+  (0: pseudo-random, 1: the one nearest the centroid class first);
+* ``single_tester`` -- whether an edge between two active triangles is tested
+  by the lower one only.
 
 ``tune`` times a workload over a small grid of values by coordinate descent
 (one knob at a time, in the order above), keeps the fastest, applies it with
@@ -29,8 +33,10 @@ CANDIDATES = {
     "grid_cap": [2048, 4096, 8192, 16384],
     "flip_items": [1, 2, 4],
     "ins_items": [1, 2, 4],
+    "pivot": [0, 1],
+    "single_tester": [0, 1],
 }
-DEFAULTS = {"tail_threshold": 4096, "grid_cap": 4096, "flip_items": 1, "ins_items": 1}
+DEFAULTS = {"tail_threshold": 4096, "grid_cap": 4096, "flip_items": 1, "ins_items": 1, "pivot": 0, "single_tester": 1}
 
 
 def cache_path() -> str:
@@ -77,15 +83,19 @@ def _store(path, data):
 
 
 def _time(run, repeats):
-    best = float("inf")
+    """Median of `repeats` timed runs: the flip phase's last rounds vary from
+    run to run (a few hundred tail rounds more or less), and a minimum would
+    rank settings by their luckiest run."""
+    ts = []
     for _ in range(repeats):
         t0 = time.perf_counter()
         run()
-        best = min(best, time.perf_counter() - t0)
-    return best
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2]
 
 
-def tune(kind: str, run, candidates=None, repeats: int = 3, key: str | None = None, path: str | None = None,
+def tune(kind: str, run, candidates=None, repeats: int = 5, key: str | None = None, path: str | None = None,
          min_gain: float = 0.02):
     """Time `run()` (one complete solve, synchronised) over the candidates and
     apply the fastest; a knob leaves its current value only for a candidate
