@@ -851,7 +851,7 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     pend2.release();
     long long Tn = 2 * NP + 2LL * K;
     DBuf<int3> tri(Tn), adj(Tn);
-    int T = filter_leaves(c, tB.p, aB.p, sB.p, nt, tri.p, adj.p, Tn);
+    int T = filter_leaves_inst(c, pts.p, vinst.p, NP, K, tB.p, aB.p, sB.p, nt, tri.p, adj.p, Tn);
     tB.release();
     aB.release();
     sB.release();

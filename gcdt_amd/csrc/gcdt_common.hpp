@@ -78,6 +78,26 @@ __device__ inline int wave_alloc(int k, int* counter)
     return base + ex;
 }
 
+// atomicAdd(&ctr[key], 1) for every lane with key >= 0, combined over runs of
+// equal keys in consecutive lanes: the first lane of a run adds the run's
+// length once and each lane gets the value its own add would have returned in
+// lane order.  Keys < 0 add nothing and get -1.  All lanes must be active.
+__device__ inline int wave_run_add(int* ctr, int key)
+{
+    int lane = __lane_id();
+    int prev = __shfl_up(key, 1, 64);
+    bool head = lane == 0 || prev != key;
+    unsigned long long heads = __ballot(head);
+    unsigned long long upto = (2ull << lane) - 1;  // lanes 0..lane (all 64 for lane 63)
+    int first = 63 - __clzll(heads & upto);
+    unsigned long long after = heads & ~upto;
+    int next = after ? __ffsll((long long)after) - 1 : 64;
+    int base = 0;
+    if (head && key >= 0) base = atomicAdd(&ctr[key], next - lane);
+    base = __shfl(base, first, 64);
+    return key >= 0 ? base + lane - first : -1;
+}
+
 // Block-wide reservation of k entries per thread with one atomicAdd per block.
 // Every thread of the block must call it (block-uniform control flow).
 template <int BS>

@@ -366,6 +366,7 @@ __global__ __launch_bounds__(1024) void k_scan_sums(int* sums, int nt, int* tota
     }
 }
 
+template <bool INCL>
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int* out, long long n, const int* sums)
 {
     __shared__ int tile[SCAN_PAD(SCAN_TILE)];
@@ -388,8 +389,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int* out, 
     int run = sums[blockIdx.x] + block_excl_scan(s, tot);
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; j++) {
+        run += INCL ? v[j] : 0;
         tile[SCAN_PAD(threadIdx.x * SCAN_ITEMS + j)] = run;
-        run += v[j];
+        run += INCL ? 0 : v[j];
     }
     __syncthreads();
 #pragma unroll
@@ -402,9 +404,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const int* in, int* out, 
 
 long long scan_scratch_ints(long long n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
-// Exclusive scan of in[0..n) into out[0..n); the total goes to dcnt[C_TOTAL]
-// and, when total_at_end, also to out[n] (out must then hold n+1 ints).
-static void scan_impl(Ctx& c, const int* in, int* out, long long n, bool total_at_end)
+// Exclusive (inclusive when incl) scan of in[0..n) into out[0..n); the total
+// goes to dcnt[C_TOTAL] and, when total_at_end, also to out[n] (out must then
+// hold n+1 ints).
+static void scan_impl(Ctx& c, const int* in, int* out, long long n, bool total_at_end, bool incl = false)
 {
     if (n <= 0) {
         c.set(C_TOTAL, 0);
@@ -428,11 +431,15 @@ static void scan_impl(Ctx& c, const int* in, int* out, long long n, bool total_a
     GCDT_LAUNCH_CHECK();
     k_scan_sums<<<1, 1024, 0, c.st>>>(sums, (int)nt, c.dcnt.p + C_TOTAL, total_at_end ? out + n : nullptr);
     GCDT_LAUNCH_CHECK();
-    k_scan_apply<<<(unsigned)nt, kBlock, 0, c.st>>>(in, out, n, sums);
+    if (incl)
+        k_scan_apply<true><<<(unsigned)nt, kBlock, 0, c.st>>>(in, out, n, sums);
+    else
+        k_scan_apply<false><<<(unsigned)nt, kBlock, 0, c.st>>>(in, out, n, sums);
     GCDT_LAUNCH_CHECK();
 }
 
 void scan_excl(Ctx& c, const int* in, int* out, long long n) { scan_impl(c, in, out, n, false); }
+void scan_incl(Ctx& c, const int* in, int* out, long long n) { scan_impl(c, in, out, n, false, true); }
 void scan_excl_total(Ctx& c, const int* in, int* out, long long n) { scan_impl(c, in, out, n, true); }
 
 // ================================================================== square / sample
@@ -585,18 +592,24 @@ __device__ inline unsigned morton_cell(int2 p)
     return x | (y << 1);
 }
 
-__global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n, int* cnt)
+// Counting sort in two passes with one atomic per point: the count pass keeps
+// the value its atomicAdd returns (the point's rank within its cell) in
+// inv[], and the scatter places the point at the cell's start plus that rank.
+__global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n, int* cnt, int* rank)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[morton_cell(pts[i])], 1);
+        rank[i] = atomicAdd(&cnt[morton_cell(pts[i])], 1);
 }
 
-__global__ __launch_bounds__(kBlock) void k_morton_scatter(const int2* pts, int n, int* cursor, int2* out, int* perm,
-                                                           int* inv)
+// cell_end holds the inclusive scan of the counts: cell c starts at
+// cell_end[c - 1] (0 for c = 0).
+__global__ __launch_bounds__(kBlock) void k_morton_scatter(const int2* pts, int n, const int* cell_end, int2* out,
+                                                           int* perm, int* inv)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         int2 p = pts[i];
-        int slot = atomicAdd(&cursor[morton_cell(p)], 1);
+        int cell = (int)morton_cell(p);
+        int slot = (cell ? cell_end[cell - 1] : 0) + inv[i];
         out[slot] = p;
         perm[slot] = (int)i;
         inv[i] = slot;
@@ -607,10 +620,9 @@ void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv
 {
     fill_i32(c, cell_end, 0, BITMAP_CELLS + 1);
     if (n <= 0) return;
-    k_morton_count<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end);
+    k_morton_count<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv);
     GCDT_LAUNCH_CHECK();
-    scan_excl(c, cell_end, cell_end, BITMAP_CELLS);
-    // the scatter advances each cell's cursor to the cell's end
+    scan_incl(c, cell_end, cell_end, BITMAP_CELLS);
     k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, out, perm, inv);
     GCDT_LAUNCH_CHECK();
 }
@@ -1495,14 +1507,22 @@ __global__ __launch_bounds__(kBlock) void k_batch_out(const int3* tri, int T, co
                                                       const long long* poff, const long long* toff, long long NP,
                                                       int* cursor, int3* out, int* err)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
-        int3 t = tri[i];
-        int inst = t.x < NP ? vinst[t.x] : (int)((t.x - NP) >> 2);
+    WAVE_LOOP(i, T)
+    {
+        int3 t = make_int3(0, 0, 0);
+        int inst = -1;
+        if (i < T) {
+            t = tri[i];
+            inst = t.x < NP ? vinst[t.x] : (int)((t.x - NP) >> 2);
+        }
+        // triangles of one instance sit together in the mesh: one atomic per run
+        int r = wave_run_add(cursor, inst);
+        if (inst < 0) continue;
         long long base = poff[inst], n = poff[inst + 1] - poff[inst];
         auto local = [&](int v) -> int {
             return v < NP ? (int)((perm ? perm[v] : v) - base) : (int)(n + ((v - NP) & 3));
         };
-        long long pos = toff[inst] + atomicAdd(&cursor[inst], 1);
+        long long pos = toff[inst] + r;
         if (pos >= toff[inst + 1]) {
             atomicOr(err, MERR_INTERNAL_COUNT);
             continue;
@@ -1521,20 +1541,24 @@ void batch_vinst(Ctx& c, const long long* off, int K, long long NP, int* vinst)
 // Batch points counting-sorted by (instance, 16x16 cell): instance ranges are
 // kept (the key is instance-major), neighbours in the plane become neighbours
 // in memory, and every cell's range start is a sample point.
-__global__ __launch_bounds__(kBlock) void k_batch_cell_count(const int2* pts, const int* vinst, long long NP, int* cnt)
+// The same one-atomic counting sort as morton_order: ranks in inv[], then an
+// inclusive scan, then a gather of each cell's start.
+__global__ __launch_bounds__(kBlock) void k_batch_cell_count(const int2* pts, const int* vinst, long long NP, int* cnt,
+                                                             int* rank)
 {
     GridInst g{vinst, NP};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[pt_cell(g, (int)i, pts[i])], 1);
+        rank[i] = atomicAdd(&cnt[pt_cell(g, (int)i, pts[i])], 1);
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_cell_scatter(const int2* pts, const int* vinst, long long NP,
-                                                               int* cursor, int2* out, int* perm, int* inv)
+                                                               const int* cell_end, int2* out, int* perm, int* inv)
 {
     GridInst g{vinst, NP};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
         int2 p = pts[i];
-        int slot = atomicAdd(&cursor[pt_cell(g, (int)i, p)], 1);
+        int cell = pt_cell(g, (int)i, p);
+        int slot = (cell ? cell_end[cell - 1] : 0) + inv[i];
         out[slot] = p;
         perm[slot] = (int)i;
         inv[i] = slot;
@@ -1547,10 +1571,9 @@ void batch_order(Ctx& c, const int2* pts, const int* vinst, long long NP, int K,
     long long ncells = (long long)K << (2 * GridInst::LOG);
     fill_i32(c, cell_end, 0, ncells + 1);
     if (NP <= 0) return;
-    k_batch_cell_count<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, cell_end);
+    k_batch_cell_count<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, cell_end, inv);
     GCDT_LAUNCH_CHECK();
-    scan_excl(c, cell_end, cell_end, ncells);
-    // the scatter advances each cell's cursor to the cell's end
+    scan_incl(c, cell_end, cell_end, ncells);
     k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, cell_end, out, perm, inv);
     GCDT_LAUNCH_CHECK();
 }
@@ -1681,25 +1704,36 @@ struct LeafKeyOne {
     __device__ int key(const int3&, int2 g) const { return (int)morton_cell(g); }
 };
 
+// The count pass keeps each leaf's rank within its cell in rank[]; the dest
+// pass adds the cell's start to it without a second atomic.
 template <class KEY>
 __global__ __launch_bounds__(kBlock) void k_leaf_cells(KEY kf, const int2* pts, const int3* tri, const int* sons, int T,
-                                                       int* cnt, int* cellof)
+                                                       int* cnt, int* cellof, int* rank)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
-        if (sons[i] != -1) continue;
-        int3 t = tri[i];
-        int2 a = pts[t.x], b = pts[t.y], c = pts[t.z];
-        int2 g = make_int2((int)(((long long)a.x + b.x + c.x) / 3), (int)(((long long)a.y + b.y + c.y) / 3));
-        int cell = kf.key(t, g);
-        cellof[i] = cell;
-        atomicAdd(&cnt[cell], 1);
+    WAVE_LOOP(i, T)
+    {
+        int cell = -1;
+        if (i < T && sons[i] == -1) {
+            int3 t = tri[i];
+            int2 a = pts[t.x], b = pts[t.y], c = pts[t.z];
+            int2 g = make_int2((int)(((long long)a.x + b.x + c.x) / 3), (int)(((long long)a.y + b.y + c.y) / 3));
+            cell = kf.key(t, g);
+        }
+        // consecutive leaves were allocated together and mostly share a cell
+        int r = wave_run_add(cnt, cell);
+        if (cell >= 0) {
+            cellof[i] = cell;
+            rank[i] = r;
+        }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_leaf_dest(const int* sons, const int* cellof, int T, int* cursor, int* dest)
+// dest[] holds the ranks on entry and the destinations on exit.
+__global__ __launch_bounds__(kBlock) void k_leaf_dest(const int* sons, const int* cellof, int T, const int* start,
+                                                      int* dest)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x)
-        dest[i] = sons[i] == -1 ? atomicAdd(&cursor[cellof[i]], 1) : -1;
+        dest[i] = sons[i] == -1 ? start[cellof[i]] + dest[i] : -1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_leaf_move(const int3* tri, const int3* adj, const int* dest, int T, int3* tri2,
@@ -1727,7 +1761,7 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
 {
     DBuf<int> cnt((size_t)ncells + 1), cellof(T > 0 ? T : 1), dest(T > 0 ? T : 1);
     fill_i32(c, cnt.p, 0, ncells + 1);
-    k_leaf_cells<KEY><<<grid_for(T), kBlock, 0, c.st>>>(kf, pts, tri, sons, T, cnt.p, cellof.p);
+    k_leaf_cells<KEY><<<grid_for(T), kBlock, 0, c.st>>>(kf, pts, tri, sons, T, cnt.p, cellof.p, dest.p);
     GCDT_LAUNCH_CHECK();
     scan_excl(c, cnt.p, cnt.p, ncells);
     int n = c.get(C_TOTAL);
@@ -1743,6 +1777,24 @@ int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* a
                          int3* adj2, long long cap)
 {
     return filter_leaves_sorted(c, LeafKeyOne{}, BITMAP_CELLS, pts, tri, adj, sons, T, tri2, adj2, cap);
+}
+
+// Batch meshes: instance-major, then the Morton order of the centroid's
+// 16x16 cell over the instance's square.
+struct LeafKeyInst {
+    const int* vinst;
+    long long NP;  // corners of instance i are NP + 4i .. NP + 4i + 3
+    __device__ int key(const int3& t, int2 g) const
+    {
+        int inst = t.x < NP ? vinst[t.x] : (int)((t.x - NP) >> 2);
+        return (inst << 8) | (int)(morton_cell(g) >> 12);
+    }
+};
+
+int filter_leaves_inst(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
+                       const int3* adj, const int* sons, int T, int3* tri2, int3* adj2, long long cap)
+{
+    return filter_leaves_sorted(c, LeafKeyInst{vinst, NP}, (long long)K << 8, pts, tri, adj, sons, T, tri2, adj2, cap);
 }
 
 int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2, int3* adj2,

@@ -105,7 +105,7 @@ enum : int {
 // GCDT_TAIL_THRESHOLD, GCDT_PIVOT, GCDT_SINGLE_TESTER override them at start).
 struct Tunables {
     int tail_threshold = 4096;
-    int pivot = 0;
+    int pivot = 1;
     int single_tester = 1;
     int flip_items = 1;  // items per thread of flip test/apply: 1, 2 or 4
     int ins_items = 1;   // items per thread of insertion split/relocate: 1, 2 or 4
@@ -126,7 +126,7 @@ struct Ctx {
     int tail_threshold = 4096;
     // tunables (environment overrides for experiments): insertion pivot policy
     // and whether both-active edges are tested by the lower triangle only
-    int pivot = 0;
+    int pivot = 1;
     int single_tester = 1;
     int flip_items = 1;
     int ins_items = 1;
@@ -179,6 +179,8 @@ void fill_i32(Ctx& c, int* p, int v, long long n);
 void fill_u64(Ctx& c, unsigned long long* p, unsigned long long v, long long n);
 // exclusive scan of n ints (out may alias in); total also to dcnt[C_TOTAL]
 void scan_excl(Ctx& c, const int* in, int* out, long long n);
+// inclusive scan: out[i] = in[0] + ... + in[i]
+void scan_incl(Ctx& c, const int* in, int* out, long long n);
 // same, and out[n] = total (out holds n+1 ints)
 void scan_excl_total(Ctx& c, const int* in, int* out, long long n);
 long long scan_scratch_ints(long long n);
@@ -236,6 +238,9 @@ int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int
 // same, with the leaves placed in Morton order of their centroid cell
 int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2,
                          int3* adj2, long long cap);
+// batch meshes: leaves sorted by (instance, 16x16 centroid cell)
+int filter_leaves_inst(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
+                       const int3* adj, const int* sons, int T, int3* tri2, int3* adj2, long long cap);
 // bitmap CSR over 1024^2 cells; allocates offsets/tris from the pool
 void build_bitmap(Ctx& c, const int2* pts, const int3* tri, int T, int** offsets, int** tris, int* ntris);
 // mark vertices of tri[0..T) in isv (must be zeroed), then locate the other

@@ -36,7 +36,7 @@ CANDIDATES = {
     "pivot": [0, 1],
     "single_tester": [0, 1],
 }
-DEFAULTS = {"tail_threshold": 4096, "grid_cap": 4096, "flip_items": 1, "ins_items": 1, "pivot": 0, "single_tester": 1}
+DEFAULTS = {"tail_threshold": 4096, "grid_cap": 4096, "flip_items": 1, "ins_items": 1, "pivot": 1, "single_tester": 1}
 
 
 def cache_path() -> str:
