@@ -31,16 +31,19 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 #   flip_test : tri 12 + adj 12 + cflag 4 + own 3 points 24
 #               + 3 x (neighbour tri 12 + opposite point 8) + cand 4 + 2 claims 16
 #   ins_reloc : loc 4 + sons 4 + list write 4 + 2 children tri 24 + 3 points 24 + loc write 4
-#   ins_claim : loc 4 + point id 4 + 1.x claim atomics 8
+#               + next round's claim: key write 8 + claim atomic 8 + claimed-triangle append 4
+#   ins_claim : loc 4 + point id 4 + 1.x claim atomics 8 (first round of a run only)
+#   ins_split : per claimed triangle: list 4 + claim 8 + winner loc 4 + key 8 + tri/adj 24
+#               + 3 children x (tri 12 + adj 12) written + sons 4 + loc 4
 #   flip_apply: cand 4 + tri 12 + adj 12 + 2 claims 16 (winners add 2 x 28 write)
 #   locate_bitmap: point 8 + isv 4 + 2 offsets 8 + ~1.5 x (tri id 4 + tri 12 + 3 points 24) + loc 4 + list 4
 BYTES_PER_ITEM = {
     "flip_test": 132,
-    "ins_reloc": 64,
+    "ins_reloc": 84,
     "ins_claim": 16,
     "flip_apply": 44,
     "locate_bitmap": 88,
-    "ins_split": 16,
+    "ins_split": 128,
 }
 
 

@@ -822,39 +822,85 @@ __global__ void k_round_begin(int* cnt, int move)
     if (move) cnt[C_NPEND] = cnt[C_NPEND2];
     cnt[C_NPEND2] = 0;
     cnt[C_TBASE] = cnt[C_NTRI];
+    cnt[C_NW] = cnt[C_NW2];
+    cnt[C_NW2] = 0;
+}
+
+// The claimed-triangle list of a round: the claim whose atomicMax finds a key
+// of an older epoch is the triangle's first this round and appends it, so
+// every claimed triangle is listed once and the split step walks these
+// triangles instead of every pending point.
+__device__ inline bool claim_first(unsigned long long old, unsigned epoch)
+{
+    return (unsigned)(old >> 44) != (epoch & 0xfffffu);
 }
 
 // Wave-combined claims: the lanes that target the same triangle as the wave's
 // first active lane reduce their keys with shuffles and issue one atomic, so a
 // triangle holding thousands of pending points (early sample rounds, dense
-// clusters) does not serialise on one address.
-__device__ inline void claim_combined(unsigned long long* claim, int t, unsigned long long key)
+// clusters) does not serialise on one address.  *f1 / *f2 report whether this
+// lane's claims on t / t2 came first this round.
+__device__ inline void claim_combined_w(unsigned long long* claim, int t, int t2, unsigned long long key,
+                                        unsigned epoch, bool* f1, bool* f2)
 {
+    *f1 = false;
+    *f2 = false;
     unsigned long long act = __ballot(t >= 0);
-    if (act == 0) return;
-    int leader = __ffsll((long long)act) - 1;
-    int tl = __shfl(t, leader, 64);
-    bool mine = t >= 0 && t == tl;
-    unsigned long long v = mine ? key : 0ull;
+    if (act != 0) {
+        int leader = __ffsll((long long)act) - 1;
+        int tl = __shfl(t, leader, 64);
+        bool mine = t >= 0 && t == tl;
+        unsigned long long v = mine ? key : 0ull;
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        unsigned long long o = __shfl_xor(v, d, 64);
-        v = o > v ? o : v;
+        for (int d = 32; d >= 1; d >>= 1) {
+            unsigned long long o = __shfl_xor(v, d, 64);
+            v = o > v ? o : v;
+        }
+        if (__lane_id() == leader) *f1 = claim_first(atomicMax(&claim[tl], v), epoch);
+        if (t >= 0 && !mine) *f1 = claim_first(atomicMax(&claim[t], key), epoch);
     }
-    if (__lane_id() == leader) atomicMax(&claim[tl], v);
-    if (t >= 0 && !mine) atomicMax(&claim[t], key);
+    if (t2 >= 0) *f2 = claim_first(atomicMax(&claim[t2], key), epoch);
 }
 
-__global__ __launch_bounds__(kBlock) void k_ins_claim(Ins s, const int* list, const int* cnt, unsigned epoch)
+// Claims of `epoch` for up to IT points per thread (p < 0: none), appending
+// first-claimed triangles to wl.  Block-uniform: every thread of the block
+// calls it.
+template <int IT>
+__device__ inline void claim_items(const Ins& s, const int (&pp)[IT], unsigned epoch, int* wl, int* nw, int wcap)
 {
-    int n = *cnt;
-    WAVE_LOOP(i, n)
-    {
+    int wt[2 * IT];
+    int nq = 0;
+#pragma unroll
+    for (int j = 0; j < IT; j++) {
         int t = -1, t2 = -1;
         unsigned long long key = 0;
-        if (i < n) t = ins_key_round(s, list[i], epoch, &key, &t2);
-        claim_combined(s.claim, t, key);
-        if (t2 >= 0) atomicMax(&s.claim[t2], key);
+        if (pp[j] >= 0) t = ins_key_round(s, pp[j], epoch, &key, &t2);
+        bool f1, f2;
+        claim_combined_w(s.claim, t, t2, key, epoch, &f1, &f2);
+        wt[2 * j] = f1 ? t : -1;  // the combining leader claims its own triangle
+        wt[2 * j + 1] = f2 ? t2 : -1;
+        nq += (wt[2 * j] >= 0) + (wt[2 * j + 1] >= 0);
+    }
+    int b = block_alloc<kBlock>(nq, nw);
+#pragma unroll
+    for (int j = 0; j < 2 * IT; j++)
+        if (wt[j] >= 0) {
+            if (b < wcap) wl[b] = wt[j];
+            else atomicOr(s.err, MERR_CAPACITY);
+            b++;
+        }
+}
+
+// First claims of a run of rounds (later rounds claim inside k_ins_reloc).
+__global__ __launch_bounds__(kBlock) void k_ins_claim_w(Ins s, const int* list, const int* cnt, unsigned epoch, int* wl,
+                                                        int* nw, int wcap)
+{
+    int n = *cnt;
+    TILE_LOOP(base, n, kBlock)
+    {
+        long long i = base + threadIdx.x;
+        int pp[1] = {i < n ? list[i] : -1};
+        claim_items<1>(s, pp, epoch, wl, nw, wcap);
     }
 }
 
@@ -868,11 +914,14 @@ void with_items(int v, F&& f)
     else f(std::integral_constant<int, 4>{});
 }
 
+// Splits for the round's claimed triangles: a triangle's claim word names its
+// winning point; an on-edge winner splits from its own triangle (loc), not
+// from the second one it claimed.
 template <int INS_ITEMS>
-__global__ __launch_bounds__(kBlock) void k_ins_split(Ins s, const int* list, const int* cnt, unsigned epoch,
+__global__ __launch_bounds__(kBlock) void k_ins_split(Ins s, const int* list, const int* cnt, int lcap, unsigned epoch,
                                                       int* ntri)
 {
-    int n = *cnt;
+    int n = min(*cnt, lcap);  // an overfull list has flagged MERR_CAPACITY
     TILE_LOOP(base, n, kBlock * INS_ITEMS)
     {
         int pp[INS_ITEMS], need[INS_ITEMS];
@@ -880,8 +929,17 @@ __global__ __launch_bounds__(kBlock) void k_ins_split(Ins s, const int* list, co
 #pragma unroll
         for (int j = 0; j < INS_ITEMS; j++) {
             long long i = base + (long long)j * kBlock + threadIdx.x;
-            pp[j] = i < n ? list[i] : -1;
-            need[j] = pp[j] >= 0 ? ins_need<false>(s, pp[j], epoch) : 0;
+            pp[j] = -1;
+            need[j] = 0;
+            if (i < n) {
+                int t = list[i];
+                int p = (int)(unsigned)s.claim[t];
+                int l = s.loc[p];
+                if (l >= 0 && (l >> 2) == t) {
+                    pp[j] = p;
+                    need[j] = ins_need<false>(s, p, epoch);
+                }
+            }
             tot += need[j];
         }
         int b = block_alloc<kBlock>(tot, ntri);
@@ -903,8 +961,11 @@ __global__ __launch_bounds__(kBlock) void k_ins_fix(Ins s, const int* cnt)
         ins_fix(s, (int)i);
 }
 
+// Relocation of the pending points after the splits, fused with the next
+// round's claims (epoch next_ep) and its claimed-triangle list.
 template <int INS_ITEMS>
-__global__ __launch_bounds__(kBlock) void k_ins_reloc(Ins s, const int* list, const int* cnt, int* list2, int* cnt2)
+__global__ __launch_bounds__(kBlock) void k_ins_reloc(Ins s, const int* list, const int* cnt, int* list2, int* cnt2,
+                                                      unsigned next_ep, int* wl, int* nw, int wcap)
 {
     int n = *cnt;
     TILE_LOOP(base, n, kBlock * INS_ITEMS)
@@ -925,6 +986,7 @@ __global__ __launch_bounds__(kBlock) void k_ins_reloc(Ins s, const int* list, co
 #pragma unroll
         for (int j = 0; j < INS_ITEMS; j++)
             if (keep[j] >= 0) list2[b++] = keep[j];
+        claim_items<INS_ITEMS>(s, keep, next_ep, wl, nw, wcap);
     }
 }
 
@@ -1007,26 +1069,38 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
         if (h[C_NPEND] > 0 && h[C_NPEND2] >= h[C_NPEND] && h[C_NTRI] == h[C_TBASE])
             throw Error(GCDT_ERR_INTERNAL, "insertion round without progress");
         // exact list lengths of the round, for per-item bytes of the timed kernels
-        prof_add_items("ins_claim", h[C_NPEND]);
-        prof_add_items("ins_split", h[C_NPEND]);
+        prof_add_items("ins_split", h[C_NW]);
         prof_add_items("ins_reloc", h[C_NPEND]);
     };
+    // Claimed-triangle lists: every pending point claims at most two
+    // triangles, and each claimed triangle is listed once per round.
+    DBuf<int> wa, wb;
+    int* W = nullptr;   // triangles claimed for the round being launched
+    int* W2 = nullptr;  // triangles claimed for the round after it
+    long long wcap = std::min<long long>(2LL * npend, s.cap) + 1;
+    if (!done && est > c.tail_threshold) {
+        wa.ensure((size_t)wcap);
+        wb.ensure((size_t)wcap);
+        W = wa.p;
+        W2 = wb.p;
+        // the first round's claims; later rounds claim inside the relocation
+        c.set(C_NW2, 0);
+        ProfScope ps("ins_claim", c.st, est);
+        k_ins_claim_w<<<grid_for(est), kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, c.epoch + 1, W2,
+                                                          c.dcnt.p + C_NW2, (int)wcap);
+        GCDT_LAUNCH_CHECK();
+    }
     while (!done && est > c.tail_threshold) {
         if (++r > 100000) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion rounds did not converge");
         unsigned ep = c.next_epoch();
         k_round_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, r > 1);
         GCDT_LAUNCH_CHECK();
-        int g = grid_for(est);
-        {
-            ProfScope ps("ins_claim", c.st, 0);
-            k_ins_claim<<<g, kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, ep);
-            GCDT_LAUNCH_CHECK();
-        }
+        std::swap(W, W2);
         with_items(c.ins_items, [&](auto it) {
             constexpr int IT = decltype(it)::value;
             ProfScope ps("ins_split", c.st, 0);
-            k_ins_split<IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, ep,
-                                                                                 c.dcnt.p + C_NTRI);
+            k_ins_split<IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(s, W, c.dcnt.p + C_NW, (int)wcap,
+                                                                                 ep, c.dcnt.p + C_NTRI);
             GCDT_LAUNCH_CHECK();
         });
         {
@@ -1037,8 +1111,8 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
         with_items(c.ins_items, [&](auto it) {
             constexpr int IT = decltype(it)::value;
             ProfScope ps("ins_reloc", c.st, 0);
-            k_ins_reloc<IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(s, P, c.dcnt.p + C_NPEND, P2,
-                                                                                 c.dcnt.p + C_NPEND2);
+            k_ins_reloc<IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(
+                s, P, c.dcnt.p + C_NPEND, P2, c.dcnt.p + C_NPEND2, ep + 1, W2, c.dcnt.p + C_NW2, (int)wcap);
             GCDT_LAUNCH_CHECK();
         });
         c.snapshot(r & 1);

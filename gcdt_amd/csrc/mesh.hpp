@@ -98,6 +98,8 @@ enum : int {
     C_TAIL_PAR = 19,  // parity: the final list is the second buffer
     C_TAIL_F = 20,    // flips done
     C_NPROP = 21,     // proposals of the current flip round
+    C_NW = 22,        // insertion: triangles claimed for the current round
+    C_NW2 = 23,       // insertion: triangles claimed for the next round
     C_NUM = 32
 };
 
