@@ -1169,8 +1169,10 @@ __global__ void k_flip_begin(int* cnt, int move)
 // Both keep the same claims protocol; apply() sets mark = epoch + 1 on the
 // two triangles of a flip.
 struct EngSoA {
+    static constexpr bool kFused = false;
     Flip f;
     __device__ bool test(int t, unsigned ep, bool all, bool single) const { return flip_test(f, t, ep, all, single); }
+    __device__ bool fixtest(int t, unsigned ep, bool all, bool single) const { return test(t, ep, all, single); }
     template <bool COH>
     __device__ int apply(int t, unsigned ep, int* nn) const
     {
@@ -1186,10 +1188,16 @@ struct EngSoA {
 };
 
 struct EngPacked {
+    // the test launch also settles the previous round's flips (flip_fixtest_p)
+    static constexpr bool kFused = true;
     FlipP f;
     __device__ bool test(int t, unsigned ep, bool all, bool single) const
     {
         return flip_test_p(f, t, ep, all, single);
+    }
+    __device__ bool fixtest(int t, unsigned ep, bool all, bool single) const
+    {
+        return flip_fixtest_p(f, t, ep, all, single);
     }
     template <bool COH>
     __device__ int apply(int t, unsigned ep, int* nn) const
@@ -1217,7 +1225,9 @@ __global__ __launch_bounds__(kBlock) void k_flip_test(E e, const int* list, cons
             long long i = base + (long long)j * kBlock + threadIdx.x;
             if (i < n) {
                 int t = list ? list[i] : (int)i;
-                if (e.test(t, epoch, list == nullptr, single != 0)) q[j] = t;
+                if (E::kFused ? e.fixtest(t, epoch, list == nullptr, single != 0)
+                              : e.test(t, epoch, list == nullptr, single != 0))
+                    q[j] = t;
             }
             nq += q[j] >= 0;
         }
@@ -1405,7 +1415,7 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         ++checked;
         prof_add_items("flip_test", (checked == 1 && !init) ? h[C_CONST] : h[C_NACT]);
         prof_add_items("flip_apply", h[C_NPROP]);
-        prof_add_items("flip_fix", h[C_NFLIP]);
+        if (!E::kFused) prof_add_items("flip_fix", h[C_NFLIP]);  // else settled by the next test launch
     };
     while (!done && est > c.tail_threshold) {
         if (++r > 1000000) throw Error(GCDT_ERR_NOT_CONVERGED, "flip rounds did not converge");
@@ -1428,7 +1438,7 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
                 GCDT_LAUNCH_CHECK();
             }
         });
-        {
+        if (!E::kFused) {
             ProfScope ps("flip_fix", c.st, 0);
             k_flip_fix<E><<<grid_for(est), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, ep);
             GCDT_LAUNCH_CHECK();
@@ -1448,6 +1458,12 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         check_round(h);
         if (h[C_NACT2] == 0) done = true;
         est = h[C_NACT2];
+        if (E::kFused && h[C_NFLIP] > 0) {
+            // settle the last round's flips (no test launch follows)
+            ProfScope ps("flip_fix", c.st, h[C_NFLIP]);
+            k_flip_fix<E><<<grid_for(h[C_NFLIP]), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, c.epoch);
+            GCDT_LAUNCH_CHECK();
+        }
     }
     if (!done) {
         // the rest in one persistent workgroup

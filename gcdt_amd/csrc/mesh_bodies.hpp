@@ -509,6 +509,76 @@ __host__ __device__ inline bool flip_test_p(const FlipP& f, int t, unsigned epoc
     return cand != 3;
 }
 
+// flip_fix of the previous round and flip_test of this one in a single pass
+// over t's record and its neighbours' (each neighbour record is loaded once
+// for both).  Runs over the round's list, which holds every triangle flipped
+// in round epoch - 1:
+// * a marked entry of t (t flipped last round) names the old neighbour nb: if
+//   nb flipped too, the edge now belongs to nb or to nb's partner; otherwise
+//   t writes the back-pointer into nb.  Only the slots that change are stored,
+//   so back-pointers written by other triangles into t are never lost.
+// * an unmarked entry may still name a triangle that flipped last round (its
+//   fix runs in this same pass): a neighbour record without both edge
+//   vertices is resolved to its partner of that flip.
+// By the end of the launch every entry and back-pointer is settled, as after
+// a separate fix launch; the last round's marks are settled by flip_fix_p.
+__host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned epoch, bool all_active,
+                                               bool single = true)
+{
+    const FRec r = rec_load(&f.r[t]);
+    const int prev = (int)epoch - 1;
+    const int cf = r.meta & 7;
+    const int ps = rec_pslot(r.meta);
+    const int partner = (r.stamp == prev && ps < 3) ? get3(r.a, ps) : -1;
+    int cand = 3;
+    for (int k = 0; k < 3; k++) {
+        const int av = get3(r.a, k);
+        const int u = get3(r.v, nx3(k)), v = get3(r.v, pv3(k));
+        int n = av;
+        FRec rn;
+        bool have = false;
+        if (adj_is_mark(av)) {
+            const int nb = adj_unmark(av);
+            rn = rec_load(&f.r[nb]);
+            n = nb;
+            if (rn.stamp == prev) {
+                if (!has2(rn.v, u, v)) {
+                    n = get3(rn.a, rec_pslot(rn.meta));
+                    rn = rec_load(&f.r[n]);
+                }
+            } else {
+                (&f.r[nb].a.x)[slot_other(rn.v, u, v)] = t;
+            }
+            (&f.r[t].a.x)[k] = n;
+            have = true;
+        }
+        if (cand != 3 || ((cf >> k) & 1) || n < 0 || n == partner) continue;
+        if (single && n < t && all_active) continue;
+        if (!have) {
+            rn = rec_load(&f.r[n]);
+            if (!all_active && !has2(rn.v, u, v)) {
+                // n flipped last round and the edge went to its partner
+                if (rn.stamp != prev) {
+                    at_or(f.err, MERR_ADJ);
+                    continue;
+                }
+                n = get3(rn.a, rec_pslot(rn.meta));
+                rn = rec_load(&f.r[n]);
+            }
+        }
+        if (single && n < t && rec_marked(rn.meta, epoch)) continue;
+        const int j = slot_other(rn.v, u, v);
+        if (incircle(r.p[0], r.p[1], r.p[2], rec_pt(rn, j)) > 0) {
+            cand = k;
+            const unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
+            at_max(&f.r[t].claim, key);
+            at_max(&f.r[n].claim, key);
+        }
+    }
+    f.r[t].meta = (r.meta & ~(3 << 3)) | (cand << 3);
+    return cand != 3;
+}
+
 // flip_apply over records: same claims protocol and return codes.  A flip
 // rewrites both records whole -- keeping the claim word, which losers may be
 // reading, at the winning key -- with the new vertices, coordinates,
