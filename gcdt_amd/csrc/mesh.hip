@@ -1290,36 +1290,53 @@ template <class E>
 __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int* lf, int* cnt, unsigned epoch0,
                                                        int max_rounds)
 {
-    __shared__ int s_n2, s_nf;
+    // round counters double-buffered by round parity: a round's pair is reset
+    // after the round's first barrier, which every thread passes only after
+    // reading the pair two rounds back
+    __shared__ int s_n2[2], s_nf[2];
     int n = cnt[C_NACT];
     unsigned ep = epoch0;
     int r = 0, err = 0, flips = 0;
+    int nf = 0;
+    if (threadIdx.x == 0) {
+        s_n2[0] = 0;
+        s_nf[0] = 0;
+    }
+    __syncthreads();
     while (n > 0 && r < max_rounds) {
-        for (int i = threadIdx.x; i < n; i += TAIL_BS) e.test(A[i], ep, false, false);
-        if (threadIdx.x == 0) {
-            s_n2 = 0;
-            s_nf = 0;
+        const int q = r & 1;
+        // fused engines settle the previous round's flips while testing
+        for (int i = threadIdx.x; i < n; i += TAIL_BS) {
+            if (E::kFused) e.fixtest(A[i], ep, false, false);
+            else e.test(A[i], ep, false, false);
         }
         __syncthreads();
+        if (threadIdx.x == 0) {
+            s_n2[q ^ 1] = 0;
+            s_nf[q ^ 1] = 0;
+        }
         for (int i = threadIdx.x; i < n; i += TAIL_BS) {
             int t = A[i], nn = -1;
             int res = e.template apply<true>(t, ep, &nn);
             if (res == 1) {
-                B[atomicAdd(&s_n2, 1)] = t;
+                B[atomicAdd(&s_n2[q], 1)] = t;
                 e.set_mark(t, (int)ep + 1);
             } else if (res == 2) {
-                int b = atomicAdd(&s_n2, 2);
+                int b = atomicAdd(&s_n2[q], 2);
                 B[b] = t;
                 B[b + 1] = nn;
-                int q = atomicAdd(&s_nf, 2);
-                lf[q] = t;
-                lf[q + 1] = nn;
+                int w = atomicAdd(&s_nf[q], 2);
+                lf[w] = t;
+                lf[w + 1] = nn;
             }
         }
         __syncthreads();
-        int nf = s_nf, n2 = s_n2;
-        for (int i = threadIdx.x; i < nf; i += TAIL_BS) e.fix(lf[i], ep);
-        __syncthreads();
+        int n2 = s_n2[q];
+        nf = s_nf[q];
+        if (!E::kFused) {
+            for (int i = threadIdx.x; i < nf; i += TAIL_BS) e.fix(lf[i], ep);
+            __syncthreads();
+        }
         if (n2 > 0 && nf == 0) {
             err = 1;
             break;
@@ -1331,6 +1348,11 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int*
         B = tmp;
         ep++;
         r++;
+    }
+    if (E::kFused && n > 0 && nf > 0) {
+        // stopped at the round limit with the last round's flips unsettled
+        for (int i = threadIdx.x; i < nf; i += TAIL_BS) e.fix(lf[i], ep - 1);
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
         cnt[C_TAIL_N] = n;
@@ -1458,8 +1480,9 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         check_round(h);
         if (h[C_NACT2] == 0) done = true;
         est = h[C_NACT2];
-        if (E::kFused && h[C_NFLIP] > 0) {
-            // settle the last round's flips (no test launch follows)
+        if (E::kFused && done && h[C_NFLIP] > 0) {
+            // settle the last round's flips (no test follows; the tail's first
+            // test settles them when the tail runs)
             ProfScope ps("flip_fix", c.st, h[C_NFLIP]);
             k_flip_fix<E><<<grid_for(h[C_NFLIP]), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, c.epoch);
             GCDT_LAUNCH_CHECK();
