@@ -835,31 +835,29 @@ __device__ inline bool claim_first(unsigned long long old, unsigned epoch)
     return (unsigned)(old >> 44) != (epoch & 0xfffffu);
 }
 
-// Wave-combined claims: the lanes that target the same triangle as the wave's
-// first active lane reduce their keys with shuffles and issue one atomic, so a
-// triangle holding thousands of pending points (early sample rounds, dense
-// clusters) does not serialise on one address.  *f1 / *f2 report whether this
-// lane's claims on t / t2 came first this round.
+// Wave-combined claims: consecutive lanes that target the same triangle form
+// a run (the pending list stays in near-Morton order, so early rounds have
+// long runs); a segmented max over each run leaves the run's highest key in
+// its first lane, which issues the run's only atomic.  *f1 / *f2 report
+// whether this lane's claims on t / t2 came first this round.  All lanes must
+// be active.
 __device__ inline void claim_combined_w(unsigned long long* claim, int t, int t2, unsigned long long key,
                                         unsigned epoch, bool* f1, bool* f2)
 {
-    *f1 = false;
-    *f2 = false;
-    unsigned long long act = __ballot(t >= 0);
-    if (act != 0) {
-        int leader = __ffsll((long long)act) - 1;
-        int tl = __shfl(t, leader, 64);
-        bool mine = t >= 0 && t == tl;
-        unsigned long long v = mine ? key : 0ull;
+    const int lane = __lane_id();
+    const int prev = __shfl_up(t, 1, 64);
+    const bool head = lane == 0 || prev != t;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long after = heads & ~((2ull << lane) - 1);
+    const int end = after ? __ffsll((long long)after) - 2 : 63;  // last lane of this lane's run
+    unsigned long long v = t >= 0 ? key : 0ull;
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            unsigned long long o = __shfl_xor(v, d, 64);
-            v = o > v ? o : v;
-        }
-        if (__lane_id() == leader) *f1 = claim_first(atomicMax(&claim[tl], v), epoch);
-        if (t >= 0 && !mine) *f1 = claim_first(atomicMax(&claim[t], key), epoch);
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned long long o = __shfl_down(v, d, 64);
+        if (lane + d <= end) v = o > v ? o : v;
     }
-    if (t2 >= 0) *f2 = claim_first(atomicMax(&claim[t2], key), epoch);
+    *f1 = head && t >= 0 && claim_first(atomicMax(&claim[t], v), epoch);
+    *f2 = t2 >= 0 && claim_first(atomicMax(&claim[t2], key), epoch);
 }
 
 // Claims of `epoch` for up to IT points per thread (p < 0: none), appending
@@ -877,7 +875,7 @@ __device__ inline void claim_items(const Ins& s, const int (&pp)[IT], unsigned e
         if (pp[j] >= 0) t = ins_key_round(s, pp[j], epoch, &key, &t2);
         bool f1, f2;
         claim_combined_w(s.claim, t, t2, key, epoch, &f1, &f2);
-        wt[2 * j] = f1 ? t : -1;  // the combining leader claims its own triangle
+        wt[2 * j] = f1 ? t : -1;
         wt[2 * j + 1] = f2 ? t2 : -1;
         nq += (wt[2 * j] >= 0) + (wt[2 * j + 1] >= 0);
     }
