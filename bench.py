@@ -37,6 +37,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 #               + 3 children x (tri 12 + adj 12) written + sons 4 + loc 4
 #   flip_apply: cand 4 + tri 12 + adj 12 + 2 claims 16 (winners add 2 x 28 write)
 #   locate_bitmap: point 8 + isv 4 + 2 offsets 8 + ~1.5 x (tri id 4 + tri 12 + 3 points 24) + loc 4 + list 4
+# With the packed records, each flip_test launch also settles the previous
+# round's flips (listed as flip_test_fix items): per settled triangle, 2 x
+# (neighbour adj 12 + stamp 4 + meta 4) read + 2 own slots and 2 back-pointers
+# written (16); its other reads are the test's own.
+BYTES_PER_FIX_IN_TEST = 56
 BYTES_PER_ITEM = {
     "flip_test": 132,
     "ins_reloc": 84,
@@ -186,7 +191,9 @@ def main(argv=None):
         ms, launches, items = _prof(L, dom)
         check(L.gcdt_prof_only(None), "prof")
         avg_ms = ms / max(launches, 1)
-        bytes_per_launch = BYTES_PER_ITEM[dom] * items / max(launches, 1)
+        fix = _prof(L, "flip_test_fix") if dom == "flip_test" else None
+        fix_items = fix[2] if fix else 0
+        bytes_per_launch = (BYTES_PER_ITEM[dom] * items + BYTES_PER_FIX_IN_TEST * fix_items) / max(launches, 1)
         achieved = bytes_per_launch / (avg_ms / 1000.0) / 1e9 if avg_ms > 0 else 0.0
         tpi = _traffic_per_item().get(dom)
         traffic = round(tpi * items / max(launches, 1)) if tpi else None
@@ -196,7 +203,9 @@ def main(argv=None):
                 "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                 "avg_launch_us": round(avg_ms * 1000.0, 3), "launches_per_step": launches // args.steps,
                 "kernel_ms_per_step": round(ms / args.steps, 3),
-                "bytes_per_item": BYTES_PER_ITEM[dom], "items_per_launch": items // max(launches, 1)}
+                "bytes_per_item": BYTES_PER_ITEM[dom], "items_per_launch": items // max(launches, 1),
+                "settled_flips_per_launch": fix_items // max(launches, 1),
+                "bytes_per_settled_flip": BYTES_PER_FIX_IN_TEST if fix_items else None}
     # whole-solve algorithmic bytes: points in, segments in, triangles out
     b_alg = 8 * n + 8 * c + 12 * T
     solve_roof = {"bytes": b_alg, "achieved_GBs": round(b_alg / (ms_per_step / 1000.0) / 1e9, 3),

@@ -783,16 +783,31 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     check_points(c, (const int2*)points, (int)NP);
     DBuf<int> vinst(NP > 0 ? NP : 1);
     batch_vinst(c, d_poff.p, K, NP, vinst.p);
-    // points in (instance, cell) order for locality; ids are mapped back at the end
+    // points in (instance, Morton cell) order for locality, with about one cell
+    // per point (4^L cells, L in [4, 10]); ids are mapped back at the end
+    std::vector<int> lvl(K), cbase(K);
+    long long ncells = 0;
+    for (int i = 0; i < K; i++) {
+        long long n = poff[i + 1] - poff[i];
+        int L = 4;
+        while (L < 10 && (1LL << (2 * (L + 1))) <= n) L++;
+        lvl[i] = L;
+        cbase[i] = (int)ncells;
+        ncells += 1LL << (2 * L);
+    }
+    if (ncells > (1LL << 30)) return fail(GCDT_ERR_INVALID, "batch too large");
+    DBuf<int> d_lvl(K), d_cbase(K);
+    GCDT_HIP(hipMemcpyAsync(d_lvl.p, lvl.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
+    GCDT_HIP(hipMemcpyAsync(d_cbase.p, cbase.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
     DBuf<int2> pts(nv);
-    DBuf<int> perm(NP > 0 ? NP : 1), inv(NP > 0 ? NP : 1), cell_end(((size_t)K << 8) + 1);
-    batch_order(c, (const int2*)points, vinst.p, NP, K, pts.p, perm.p, inv.p, cell_end.p);
+    DBuf<int> perm(NP > 0 ? NP : 1), inv(NP > 0 ? NP : 1), cell_end((size_t)ncells + 1);
+    batch_order(c, (const int2*)points, vinst.p, d_cbase.p, d_lvl.p, NP, ncells, pts.p, perm.p, inv.p, cell_end.p);
     // ---- 1. sample meshes: one point per (instance, 16x16 cell), inserted from
     //         every instance's two seed triangles, then made Delaunay
     Timer t1(c.st);
     long long capA = 2LL * K + 4LL * std::min<long long>(NP, (long long)K << 8) + 8;
     DBuf<int> sid(std::max<long long>(1, std::min<long long>(NP, (long long)K << 8)));
-    int S = batch_sample_sorted(c, cell_end.p, K, sid.p);
+    int S = batch_sample_sorted(c, d_cbase.p, d_lvl.p, cell_end.p, K, sid.p);
     cell_end.release();
     DBuf<int3> tA(capA), aA(capA);
     DBuf<int> sA(capA), loc(nv), pend(NP > 0 ? NP : 1), pend2(NP > 0 ? NP : 1);

@@ -223,7 +223,13 @@ void prof_add_items(const char* name, long long items)
     if (!g_prof_on) return;
     std::lock_guard<std::mutex> g(g_prof_mu);
     for (auto& r : g_prof)
-        if (r.name == name) r.items += items;
+        if (r.name == name) {
+            r.items += items;
+            return;
+        }
+    // an item count without timed launches of its own (work done inside
+    // another kernel's launches)
+    g_prof.push_back(ProfRec{name, {}, 0, items});
 }
 
 void prof_only(const char* name)
@@ -1423,6 +1429,7 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         c.set(C_CONST, T);
     }
     long long flips = 0;
+    long long settled = 0;  // fused engines: flipped triangles settled inside test launches
     int r = 0;
     int est = init ? ninit : T;
     bool done = est == 0;
@@ -1432,6 +1439,7 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         // the highest-keyed proposal always wins both of its triangles
         if (h[C_NACT2] > 0 && h[C_NFLIP] == 0) throw Error(GCDT_ERR_INTERNAL, "flip round without progress");
         flips += h[C_NFLIP] / 2;
+        settled += h[C_NFLIP];
         ++checked;
         prof_add_items("flip_test", (checked == 1 && !init) ? h[C_CONST] : h[C_NACT]);
         prof_add_items("flip_apply", h[C_NPROP]);
@@ -1478,6 +1486,8 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         check_round(h);
         if (h[C_NACT2] == 0) done = true;
         est = h[C_NACT2];
+        // the last round's flips are settled by the tail (or the fix below)
+        if (E::kFused) prof_add_items("flip_test_fix", settled - h[C_NFLIP]);
         if (E::kFused && done && h[C_NFLIP] > 0) {
             // settle the last round's flips (no test follows; the tail's first
             // test settles them when the tail runs)
@@ -1649,26 +1659,37 @@ void batch_vinst(Ctx& c, const long long* off, int K, long long NP, int* vinst)
     GCDT_LAUNCH_CHECK();
 }
 
-// Batch points counting-sorted by (instance, 16x16 cell): instance ranges are
-// kept (the key is instance-major), neighbours in the plane become neighbours
-// in memory, and every cell's range start is a sample point.
-// The same one-atomic counting sort as morton_order: ranks in inv[], then an
-// inclusive scan, then a gather of each cell's start.
-__global__ __launch_bounds__(kBlock) void k_batch_cell_count(const int2* pts, const int* vinst, long long NP, int* cnt,
+// Batch points counting-sorted by instance, then by the Morton order of a
+// per-instance grid of 4^L cells (L in [4, 10], about one cell per point, the
+// instance's cells numbered from cbase[i]): instance ranges are kept,
+// neighbours in the plane become neighbours in memory down to single points,
+// and each 16x16 cell of an instance is a contiguous range whose first point
+// is a sample.  The same one-atomic counting sort as morton_order: ranks in
+// inv[], then an inclusive scan, then a gather of each cell's start.
+struct BatchCells {
+    const int* vinst;
+    const int* cbase;
+    const int* lvl;
+    __device__ int cell(int i, int2 p) const
+    {
+        int inst = vinst[i];
+        return cbase[inst] + (int)(morton_cell(p) >> (2 * (10 - lvl[inst])));
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void k_batch_cell_count(BatchCells b, const int2* pts, long long NP, int* cnt,
                                                              int* rank)
 {
-    GridInst g{vinst, NP};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x)
-        rank[i] = atomicAdd(&cnt[pt_cell(g, (int)i, pts[i])], 1);
+        rank[i] = atomicAdd(&cnt[b.cell((int)i, pts[i])], 1);
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_cell_scatter(const int2* pts, const int* vinst, long long NP,
+__global__ __launch_bounds__(kBlock) void k_batch_cell_scatter(BatchCells b, const int2* pts, long long NP,
                                                                const int* cell_end, int2* out, int* perm, int* inv)
 {
-    GridInst g{vinst, NP};
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
         int2 p = pts[i];
-        int cell = pt_cell(g, (int)i, p);
+        int cell = b.cell((int)i, p);
         int slot = (cell ? cell_end[cell - 1] : 0) + inv[i];
         out[slot] = p;
         perm[slot] = (int)i;
@@ -1676,41 +1697,55 @@ __global__ __launch_bounds__(kBlock) void k_batch_cell_scatter(const int2* pts, 
     }
 }
 
-void batch_order(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, int2* out, int* perm, int* inv,
-                 int* cell_end)
+void batch_order(Ctx& c, const int2* pts, const int* vinst, const int* cbase, const int* lvl, long long NP,
+                 long long ncells, int2* out, int* perm, int* inv, int* cell_end)
 {
-    long long ncells = (long long)K << (2 * GridInst::LOG);
     fill_i32(c, cell_end, 0, ncells + 1);
     if (NP <= 0) return;
-    k_batch_cell_count<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, cell_end, inv);
+    BatchCells b{vinst, cbase, lvl};
+    k_batch_cell_count<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv);
     GCDT_LAUNCH_CHECK();
     scan_incl(c, cell_end, cell_end, ncells);
-    k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(pts, vinst, NP, cell_end, out, perm, inv);
+    k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, out, perm, inv);
     GCDT_LAUNCH_CHECK();
 }
 
-__global__ __launch_bounds__(kBlock) void k_range_flags(const int* cell_end, long long ncells, int* flags)
+// Sorted range of 16x16 cell c (Morton index) of instance i: the fine cells
+// [cbase + (c << 2(L-4)), cbase + ((c+1) << 2(L-4))).
+__device__ inline int2 coarse_range(const int* cbase, const int* lvl, const int* cell_end, long long idx)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long long)gridDim.x * blockDim.x)
-        flags[i] = cell_end[i] > (i ? cell_end[i - 1] : 0);
+    int inst = (int)(idx >> 8), c = (int)(idx & 255);
+    int sh = 2 * (lvl[inst] - 4);
+    int f0 = cbase[inst] + (c << sh), f1 = cbase[inst] + ((c + 1) << sh);
+    return make_int2(f0 ? cell_end[f0 - 1] : 0, cell_end[f1 - 1]);
 }
 
-__global__ __launch_bounds__(kBlock) void k_range_emit(const int* cell_end, const int* offs, long long ncells, int* ids)
+__global__ __launch_bounds__(kBlock) void k_range_flags(const int* cbase, const int* lvl, const int* cell_end,
+                                                        long long ncoarse, int* flags)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long long)gridDim.x * blockDim.x) {
-        int a = i ? cell_end[i - 1] : 0;
-        if (cell_end[i] > a) ids[offs[i]] = a;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncoarse; i += (long long)gridDim.x * blockDim.x) {
+        int2 r = coarse_range(cbase, lvl, cell_end, i);
+        flags[i] = r.y > r.x;
     }
 }
 
-int batch_sample_sorted(Ctx& c, const int* cell_end, int K, int* ids)
+__global__ __launch_bounds__(kBlock) void k_range_emit(const int* cbase, const int* lvl, const int* cell_end,
+                                                       const int* offs, long long ncoarse, int* ids)
 {
-    long long ncells = (long long)K << (2 * GridInst::LOG);
-    DBuf<int> offs((size_t)ncells + 1);
-    k_range_flags<<<grid_for(ncells), kBlock, 0, c.st>>>(cell_end, ncells, offs.p);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < ncoarse; i += (long long)gridDim.x * blockDim.x) {
+        int2 r = coarse_range(cbase, lvl, cell_end, i);
+        if (r.y > r.x) ids[offs[i]] = r.x;
+    }
+}
+
+int batch_sample_sorted(Ctx& c, const int* cbase, const int* lvl, const int* cell_end, int K, int* ids)
+{
+    long long ncoarse = (long long)K << 8;
+    DBuf<int> offs((size_t)ncoarse + 1);
+    k_range_flags<<<grid_for(ncoarse), kBlock, 0, c.st>>>(cbase, lvl, cell_end, ncoarse, offs.p);
     GCDT_LAUNCH_CHECK();
-    scan_excl(c, offs.p, offs.p, ncells);
-    k_range_emit<<<grid_for(ncells), kBlock, 0, c.st>>>(cell_end, offs.p, ncells, ids);
+    scan_excl(c, offs.p, offs.p, ncoarse);
+    k_range_emit<<<grid_for(ncoarse), kBlock, 0, c.st>>>(cbase, lvl, cell_end, offs.p, ncoarse, ids);
     GCDT_LAUNCH_CHECK();
     return c.get(C_TOTAL);
 }

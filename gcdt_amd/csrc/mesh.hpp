@@ -213,12 +213,13 @@ void unmap_tris(Ctx& c, int3* tri, int T, const int* perm, int n);
 // constraint ids -> global, and the per-instance output scatter
 void batch_vinst(Ctx& c, const long long* off, int K, long long NP, int* vinst);
 void batch_init(Ctx& c, int2* pts, int3* tri, int3* adj, int K, long long NP);
-// points counting-sorted by (instance, 16x16 cell); cell_end[K*256 + 1]
+// points counting-sorted by instance, then by the Morton cell of a 4^lvl[i]
+// grid (cells of instance i numbered from cbase[i]); cell_end[ncells + 1]
 // receives each cell's end offset
-void batch_order(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, int2* out, int* perm, int* inv,
-                 int* cell_end);
-// the first point of every non-empty cell of batch_order's output
-int batch_sample_sorted(Ctx& c, const int* cell_end, int K, int* ids);
+void batch_order(Ctx& c, const int2* pts, const int* vinst, const int* cbase, const int* lvl, long long NP,
+                 long long ncells, int2* out, int* perm, int* inv, int* cell_end);
+// the first point of every non-empty 16x16 cell of batch_order's output
+int batch_sample_sorted(Ctx& c, const int* cbase, const int* lvl, const int* cell_end, int K, int* ids);
 // listed points into their instance's seed triangles
 void batch_locate_ids(Ctx& c, const int2* pts, const int* vinst, const int* ids, int n, int* loc, int* pend);
 // bitmap of a batch's sample meshes: 16x16 cells per instance; offsets has
