@@ -759,8 +759,12 @@ __global__ __launch_bounds__(kBlock) void k_locate_bitmap(G g, const int2* pts, 
                                                           const int* bm_off, const int* bm_tri, int* loc, int* pend,
                                                           int* npend, int* err)
 {
-    WAVE_LOOP(i, n)
+    __shared__ int s_buf[STAGE_CAP];
+    __shared__ int s_base;
+    Staged<STAGE_CAP> sp(s_buf, &s_base, pend, npend);
+    TILE_LOOP(base, n, kBlock)
     {
+        long long i = base + threadIdx.x;
         bool want = false;
         if (i < n) {
             int p = (int)i;
@@ -785,9 +789,10 @@ __global__ __launch_bounds__(kBlock) void k_locate_bitmap(G g, const int2* pts, 
                 }
             }
         }
-        int slot = wave_alloc(want ? 1 : 0, npend);
-        if (want) pend[slot] = (int)i;
+        int slot = sp.reserve(want ? 1 : 0);
+        if (want) s_buf[slot] = (int)i;
     }
+    sp.flush();
 }
 
 template <class G>
@@ -870,7 +875,7 @@ __device__ inline void claim_combined_w(unsigned long long* claim, int t, int t2
 // first-claimed triangles to wl.  Block-uniform: every thread of the block
 // calls it.
 template <int IT>
-__device__ inline void claim_items(const Ins& s, const int (&pp)[IT], unsigned epoch, int* wl, int* nw, int wcap)
+__device__ inline void claim_items(const Ins& s, const int (&pp)[IT], unsigned epoch, Staged<STAGE_CAP>& sw)
 {
     int wt[2 * IT];
     int nq = 0;
@@ -885,27 +890,27 @@ __device__ inline void claim_items(const Ins& s, const int (&pp)[IT], unsigned e
         wt[2 * j + 1] = f2 ? t2 : -1;
         nq += (wt[2 * j] >= 0) + (wt[2 * j + 1] >= 0);
     }
-    int b = block_alloc<kBlock>(nq, nw);
+    int b = sw.reserve(nq);
 #pragma unroll
     for (int j = 0; j < 2 * IT; j++)
-        if (wt[j] >= 0) {
-            if (b < wcap) wl[b] = wt[j];
-            else atomicOr(s.err, MERR_CAPACITY);
-            b++;
-        }
+        if (wt[j] >= 0) sw.buf[b++] = wt[j];
 }
 
 // First claims of a run of rounds (later rounds claim inside k_ins_reloc).
 __global__ __launch_bounds__(kBlock) void k_ins_claim_w(Ins s, const int* list, const int* cnt, unsigned epoch, int* wl,
                                                         int* nw, int wcap)
 {
+    __shared__ int s_buf[STAGE_CAP];
+    __shared__ int s_base;
+    Staged<STAGE_CAP> sw(s_buf, &s_base, wl, nw, wcap, s.err, MERR_CAPACITY);
     int n = *cnt;
     TILE_LOOP(base, n, kBlock)
     {
         long long i = base + threadIdx.x;
         int pp[1] = {i < n ? list[i] : -1};
-        claim_items<1>(s, pp, epoch, wl, nw, wcap);
+        claim_items<1>(s, pp, epoch, sw);
     }
+    sw.flush();
 }
 
 // Items per thread of the block-appending list kernels: 1, 2 or 4 (tunable;
@@ -971,6 +976,9 @@ template <int INS_ITEMS>
 __global__ __launch_bounds__(kBlock) void k_ins_reloc(Ins s, const int* list, const int* cnt, int* list2, int* cnt2,
                                                       unsigned next_ep, int* wl, int* nw, int wcap)
 {
+    __shared__ int s_buf[STAGE_CAP], s_buf2[STAGE_CAP];
+    __shared__ int s_base, s_base2;
+    Staged<STAGE_CAP> sl(s_buf, &s_base, list2, cnt2), sw(s_buf2, &s_base2, wl, nw, wcap, s.err, MERR_CAPACITY);
     int n = *cnt;
     TILE_LOOP(base, n, kBlock * INS_ITEMS)
     {
@@ -986,12 +994,14 @@ __global__ __launch_bounds__(kBlock) void k_ins_reloc(Ins s, const int* list, co
             }
             tot += keep[j] >= 0;
         }
-        int b = block_alloc<kBlock>(tot, cnt2);
+        int b = sl.reserve(tot);
 #pragma unroll
         for (int j = 0; j < INS_ITEMS; j++)
-            if (keep[j] >= 0) list2[b++] = keep[j];
-        claim_items<INS_ITEMS>(s, keep, next_ep, wl, nw, wcap);
+            if (keep[j] >= 0) s_buf[b++] = keep[j];
+        claim_items<INS_ITEMS>(s, keep, next_ep, sw);
     }
+    sl.flush();
+    sw.flush();
 }
 
 // ---- persistent single-workgroup tail: all remaining insertion rounds in one
@@ -1218,6 +1228,9 @@ template <class E, int FT_ITEMS>
 __global__ __launch_bounds__(kBlock) void k_flip_test(E e, const int* list, const int* cnt, unsigned epoch, int single,
                                                       int* prop, int* nprop)
 {
+    __shared__ int s_buf[STAGE_CAP];
+    __shared__ int s_base;
+    Staged<STAGE_CAP> sp(s_buf, &s_base, prop, nprop);
     int n = *cnt;
     TILE_LOOP(base, n, kBlock * FT_ITEMS)
     {
@@ -1235,17 +1248,21 @@ __global__ __launch_bounds__(kBlock) void k_flip_test(E e, const int* list, cons
             }
             nq += q[j] >= 0;
         }
-        int b = block_alloc<kBlock>(nq, nprop);
+        int b = sp.reserve(nq);
 #pragma unroll
         for (int j = 0; j < FT_ITEMS; j++)
-            if (q[j] >= 0) prop[b++] = q[j];
+            if (q[j] >= 0) s_buf[b++] = q[j];
     }
+    sp.flush();
 }
 
 template <class E, int FA_ITEMS>
 __global__ __launch_bounds__(kBlock) void k_flip_apply(E e, const int* list, const int* cnt, unsigned epoch, int* list2,
                                                        int* cnt2, int* flipped, int* nflip)
 {
+    __shared__ int s_buf[STAGE_CAP], s_buf2[STAGE_CAP];
+    __shared__ int s_base, s_base2;
+    Staged<STAGE_CAP> sq(s_buf, &s_base, list2, cnt2), sf(s_buf2, &s_base2, flipped, nflip);
     int n = *cnt;
     TILE_LOOP(base, n, kBlock * FA_ITEMS)
     {
@@ -1267,18 +1284,20 @@ __global__ __launch_bounds__(kBlock) void k_flip_apply(E e, const int* list, con
             nq += (qa[j] >= 0) + (qb[j] >= 0);
             nf += (qb[j] >= 0) ? 2 : 0;
         }
-        int b = block_alloc<kBlock>(nq, cnt2);
-        int fb = block_alloc<kBlock>(nf, nflip);
+        int b = sq.reserve(nq);
+        int fb = sf.reserve(nf);
 #pragma unroll
         for (int j = 0; j < FA_ITEMS; j++) {
-            if (qa[j] >= 0) list2[b++] = qa[j];
+            if (qa[j] >= 0) s_buf[b++] = qa[j];
             if (qb[j] >= 0) {
-                list2[b++] = qb[j];
-                flipped[fb++] = qa[j];
-                flipped[fb++] = qb[j];
+                s_buf[b++] = qb[j];
+                s_buf2[fb++] = qa[j];
+                s_buf2[fb++] = qb[j];
             }
         }
     }
+    sq.flush();
+    sf.flush();
 }
 
 template <class E>
