@@ -505,7 +505,8 @@ __host__ __device__ inline bool flip_test_p(const FlipP& f, int t, unsigned epoc
             break;
         }
     }
-    f.r[t].meta = (r.meta & ~(3 << 3)) | (cand << 3);
+    const int meta = (r.meta & ~(3 << 3)) | (cand << 3);
+    if (meta != r.meta) f.r[t].meta = meta;
     return cand != 3;
 }
 
@@ -547,7 +548,9 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
                     rn = rec_load(&f.r[n]);
                 }
             } else {
-                (&f.r[nb].a.x)[slot_other(rn.v, u, v)] = t;
+                // the outer edge that kept its triangle already points here
+                const int j = slot_other(rn.v, u, v);
+                if (get3(rn.a, j) != t) (&f.r[nb].a.x)[j] = t;
             }
             (&f.r[t].a.x)[k] = n;
             have = true;
@@ -575,7 +578,10 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
             at_max(&f.r[n].claim, key);
         }
     }
-    f.r[t].meta = (r.meta & ~(3 << 3)) | (cand << 3);
+    // most tested triangles propose nothing and already say so: a store would
+    // only dirty the line
+    const int meta = (r.meta & ~(3 << 3)) | (cand << 3);
+    if (meta != r.meta) f.r[t].meta = meta;
     return cand != 3;
 }
 
@@ -634,7 +640,8 @@ __host__ __device__ inline void flip_fix_p(const FlipP& f, int t, unsigned epoch
             set3(ta, k, has2(rb.v, u, v) ? nb : get3(rb.a, rec_pslot(rb.meta)));
         } else {
             set3(ta, k, nb);
-            (&f.r[nb].a.x)[slot_other(rb.v, u, v)] = t;
+            const int j = slot_other(rb.v, u, v);
+            if (get3(rb.a, j) != t) (&f.r[nb].a.x)[j] = t;
         }
     }
     f.r[t].a = ta;
