@@ -514,15 +514,17 @@ __host__ __device__ inline bool flip_test_p(const FlipP& f, int t, unsigned epoc
 // over t's record and its neighbours' (each neighbour record is loaded once
 // for both).  Runs over the round's list, which holds every triangle flipped
 // in round epoch - 1:
-// * a marked entry of t (t flipped last round) names the old neighbour nb: if
-//   nb flipped too, the edge now belongs to nb or to nb's partner; otherwise
-//   t writes the back-pointer into nb.  Only the slots that change are stored,
-//   so back-pointers written by other triangles into t are never lost.
-// * an unmarked entry may still name a triangle that flipped last round (its
+// * if t flipped last round (stamp = epoch - 1), its two slots other than the
+//   partner's name the old neighbours nb of the flipped pair: if nb flipped
+//   too, the edge now belongs to nb or to nb's partner; otherwise t writes the
+//   back-pointer into nb.  Only slots that change are stored (most keep their
+//   value), so back-pointers written by other triangles into t are never lost
+//   and unchanged lines stay clean.
+// * any other entry may still name a triangle that flipped last round (its
 //   fix runs in this same pass): a neighbour record without both edge
 //   vertices is resolved to its partner of that flip.
 // By the end of the launch every entry and back-pointer is settled, as after
-// a separate fix launch; the last round's marks are settled by flip_fix_p.
+// a separate fix launch; the last round's flips are settled by flip_fix_p.
 __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned epoch, bool all_active,
                                                bool single = true)
 {
@@ -530,7 +532,8 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
     const int prev = (int)epoch - 1;
     const int cf = r.meta & 7;
     const int ps = rec_pslot(r.meta);
-    const int partner = (r.stamp == prev && ps < 3) ? get3(r.a, ps) : -1;
+    const bool flipped = r.stamp == prev && ps < 3;
+    const int partner = flipped ? get3(r.a, ps) : -1;
     int cand = 3;
     for (int k = 0; k < 3; k++) {
         const int av = get3(r.a, k);
@@ -538,21 +541,20 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
         int n = av;
         FRec rn;
         bool have = false;
-        if (adj_is_mark(av)) {
-            const int nb = adj_unmark(av);
-            rn = rec_load(&f.r[nb]);
-            n = nb;
+        if (flipped && k != ps && av >= 0) {
+            // an outer slot of last round's flip: it names the old neighbour
+            rn = rec_load(&f.r[av]);
             if (rn.stamp == prev) {
                 if (!has2(rn.v, u, v)) {
                     n = get3(rn.a, rec_pslot(rn.meta));
                     rn = rec_load(&f.r[n]);
+                    (&f.r[t].a.x)[k] = n;
                 }
             } else {
                 // the outer edge that kept its triangle already points here
                 const int j = slot_other(rn.v, u, v);
-                if (get3(rn.a, j) != t) (&f.r[nb].a.x)[j] = t;
+                if (get3(rn.a, j) != t) (&f.r[av].a.x)[j] = t;
             }
-            (&f.r[t].a.x)[k] = n;
             have = true;
         }
         if (cand != 3 || ((cf >> k) & 1) || n < 0 || n == partner) continue;
@@ -588,7 +590,10 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
 // flip_apply over records: same claims protocol and return codes.  A flip
 // rewrites both records whole -- keeping the claim word, which losers may be
 // reading, at the winning key -- with the new vertices, coordinates,
-// constraint bits, partner slot and mark = epoch + 1 (both are queued).
+// constraint bits, partner slot, stamp = epoch and mark = epoch + 1 (both are
+// queued).  The two outer slots of each keep the old neighbours' ids; the
+// stamp marks them for settling by the next round's test (flip_fixtest_p) or
+// by flip_fix_p.
 template <bool COH = true>
 __host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoch, int* nout)
 {
@@ -615,36 +620,35 @@ __host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoc
     const int2 pp = rec_pt(r, k), pu = rec_pt(r, kn), pv = rec_pt(r, kp), po = rec_pt(rn, j);
     const int mk = (int)epoch + 1;
     // t = (p,u,v), n = (o,v,u)  ->  t = (p,u,o) [partner in slot 1], n = (p,o,v) [partner in slot 2]
-    rec_store(&f.r[t], make_int3(p, u, o), make_int3(adj_mark(get3(rn.a, jn)), n, adj_mark(get3(r.a, kp))), pp, pu, po,
+    rec_store(&f.r[t], make_int3(p, u, o), make_int3(get3(rn.a, jn), n, get3(r.a, kp)), pp, pu, po,
               rec_with_mark(((cfn >> jn) & 1) | (((cft >> kp) & 1) << 2) | FREC_NOCAND | (1 << 5), mk), (int)epoch,
               key);
-    rec_store(&f.r[n], make_int3(p, o, v), make_int3(adj_mark(get3(rn.a, jp)), adj_mark(get3(r.a, kn)), t), pp, po, pv,
+    rec_store(&f.r[n], make_int3(p, o, v), make_int3(get3(rn.a, jp), get3(r.a, kn), t), pp, po, pv,
               rec_with_mark(((cfn >> jp) & 1) | (((cft >> kn) & 1) << 1) | FREC_NOCAND | (2 << 5), mk), (int)epoch,
               key);
     *nout = n;
     return 2;
 }
 
-// flip_fix over records.
+// flip_fix over records: settles the outer slots of t, flipped in round
+// epoch, and the back-pointers of its unflipped neighbours.
 __host__ __device__ inline void flip_fix_p(const FlipP& f, int t, unsigned epoch)
 {
     const FRec r = rec_load(&f.r[t]);
-    int3 ta = r.a;
+    const int ps = rec_pslot(r.meta);
+    if (r.stamp != (int)epoch || ps == 3) return;
     for (int k = 0; k < 3; k++) {
-        int av = get3(ta, k);
-        if (!adj_is_mark(av)) continue;
-        int nb = adj_unmark(av);
-        int u = get3(r.v, nx3(k)), v = get3(r.v, pv3(k));
+        const int nb = get3(r.a, k);
+        if (k == ps || nb < 0) continue;
+        const int u = get3(r.v, nx3(k)), v = get3(r.v, pv3(k));
         const FRec rb = rec_load(&f.r[nb]);
         if (rb.stamp == (int)epoch) {
-            set3(ta, k, has2(rb.v, u, v) ? nb : get3(rb.a, rec_pslot(rb.meta)));
+            if (!has2(rb.v, u, v)) (&f.r[t].a.x)[k] = get3(rb.a, rec_pslot(rb.meta));
         } else {
-            set3(ta, k, nb);
             const int j = slot_other(rb.v, u, v);
             if (get3(rb.a, j) != t) (&f.r[nb].a.x)[j] = t;
         }
     }
-    f.r[t].a = ta;
 }
 
 }  // namespace gcdt
