@@ -597,20 +597,24 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
 template <bool COH = true>
 __host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoch, int* nout)
 {
-    const int k = rec_cand(f.r[t].meta);
-    if (k == 3) return 0;
+    // whole records first: the proposal and both claim words lie in the two
+    // lines the flip needs anyway (inside a persistent workgroup the claims
+    // are re-read with atomic loads)
     const FRec r = rec_load(&f.r[t]);
+    const int k = rec_cand(r.meta);
+    if (k == 3) return 0;
     const int n = get3(r.a, k);
+    const FRec rn = rec_load(&f.r[n]);
     const unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
     const unsigned long long ct = COH ? ld_claim<COH>(&f.r[t].claim) : r.claim;
-    if (ct != key || ld_claim<COH>(&f.r[n].claim) != key) {
+    const unsigned long long cn = COH ? ld_claim<COH>(&f.r[n].claim) : rn.claim;
+    if (ct != key || cn != key) {
         if (ct != key) {
             int t2 = (int)((ct & 0xffffffffu) / 3);
             if ((ct >> 32) == epoch && ld_claim<COH>(&f.r[t2].claim) == ct) return 3;
         }
         return 1;
     }
-    const FRec rn = rec_load(&f.r[n]);
     const int kn = nx3(k), kp = pv3(k);
     const int p = get3(r.v, k), u = get3(r.v, kn), v = get3(r.v, kp);
     const int j = slot_other(rn.v, u, v);
