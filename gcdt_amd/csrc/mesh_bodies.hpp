@@ -534,16 +534,26 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
     const int ps = rec_pslot(r.meta);
     const bool flipped = r.stamp == prev && ps < 3;
     const int partner = flipped ? get3(r.a, ps) : -1;
+    // every neighbour record the pass can need, requested together before any
+    // store (three line reads in flight instead of a chain of three)
+    FRec nr[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int av = get3(r.a, k);
+        const bool pend = flipped && k != ps && av >= 0;
+        const bool tst = !((cf >> k) & 1) && av >= 0 && av != partner && !(single && av < t && all_active);
+        if (pend || tst) nr[k] = rec_load(&f.r[av]);
+    }
     int cand = 3;
+#pragma unroll
     for (int k = 0; k < 3; k++) {
         const int av = get3(r.a, k);
         const int u = get3(r.v, nx3(k)), v = get3(r.v, pv3(k));
         int n = av;
-        FRec rn;
+        FRec rn = nr[k];
         bool have = false;
         if (flipped && k != ps && av >= 0) {
             // an outer slot of last round's flip: it names the old neighbour
-            rn = rec_load(&f.r[av]);
             if (rn.stamp == prev) {
                 if (!has2(rn.v, u, v)) {
                     n = get3(rn.a, rec_pslot(rn.meta));
@@ -560,7 +570,6 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
         if (cand != 3 || ((cf >> k) & 1) || n < 0 || n == partner) continue;
         if (single && n < t && all_active) continue;
         if (!have) {
-            rn = rec_load(&f.r[n]);
             if (!all_active && !has2(rn.v, u, v)) {
                 // n flipped last round and the edge went to its partner
                 if (rn.stamp != prev) {
