@@ -40,6 +40,7 @@ namespace {
 struct HostSlot {
     int* hcnt;
     int* hring;
+    int* hring_dev;
     hipEvent_t ev[2];
 };
 std::mutex g_host_mu;
@@ -57,7 +58,10 @@ HostSlot take_host_slot()
     }
     HostSlot h{};
     GCDT_HIP(hipHostMalloc((void**)&h.hcnt, sizeof(int) * C_NUM, hipHostMallocDefault));
-    GCDT_HIP(hipHostMalloc((void**)&h.hring, sizeof(int) * 2 * C_NUM, hipHostMallocDefault));
+    // written by kernels (round-begin snapshots): coherent, so the writes reach
+    // host memory without a cache flush and are seen once the event completes
+    GCDT_HIP(hipHostMalloc((void**)&h.hring, sizeof(int) * 2 * C_NUM, hipHostMallocMapped | hipHostMallocCoherent));
+    GCDT_HIP(hipHostGetDevicePointer((void**)&h.hring_dev, h.hring, 0));
     GCDT_HIP(hipEventCreateWithFlags(&h.ev[0], hipEventDisableTiming));
     GCDT_HIP(hipEventCreateWithFlags(&h.ev[1], hipEventDisableTiming));
     return h;
@@ -75,6 +79,7 @@ Ctx::Ctx(hipStream_t s) : st(s)
     HostSlot h = take_host_slot();
     hcnt = h.hcnt;
     hring = h.hring;
+    hring_dev = h.hring_dev;
     ring_ev[0] = h.ev[0];
     ring_ev[1] = h.ev[1];
     if (st) {
@@ -95,15 +100,22 @@ Ctx::~Ctx()
     }
     if (hcnt) {
         std::lock_guard<std::mutex> g(g_host_mu);
-        g_host_free.push_back(HostSlot{hcnt, hring, {ring_ev[0], ring_ev[1]}});
+        g_host_free.push_back(HostSlot{hcnt, hring, hring_dev, {ring_ev[0], ring_ev[1]}});
     }
 }
 
+// the counters straight into the mapped pinned ring (a kernel costs less
+// than a copy-engine transfer of 128 bytes)
+__global__ void k_snap(const int* cnt, int* dst) { dst[threadIdx.x] = cnt[threadIdx.x]; }
+
 void Ctx::snapshot(int slot)
 {
-    GCDT_HIP(hipMemcpyAsync(hring + C_NUM * slot, dcnt.p, sizeof(int) * C_NUM, hipMemcpyDeviceToHost, st));
-    GCDT_HIP(hipEventRecord(ring_ev[slot], st));
+    k_snap<<<1, C_NUM, 0, st>>>(dcnt.p, snap_dst(slot));
+    GCDT_LAUNCH_CHECK();
+    snapshot_mark(slot);
 }
+
+void Ctx::snapshot_mark(int slot) { GCDT_HIP(hipEventRecord(ring_ev[slot], st)); }
 
 const int* Ctx::wait_snapshot(int slot)
 {
@@ -828,8 +840,13 @@ int locate_bitmap_batch(Ctx& c, const int2* pts, const int* vinst, long long NP,
 
 // Round prologue on the device: the previous round's output list becomes the
 // input (count C_NPEND2 -> C_NPEND) and the slot base is recorded.
-__global__ void k_round_begin(int* cnt, int move)
+// With snap set, it first copies the previous round's counters to the host
+// ring (one launch instead of a begin kernel plus a copy).  C_NUM threads.
+__global__ void k_round_begin(int* cnt, int move, int* snap)
 {
+    if (snap) snap[threadIdx.x] = cnt[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     if (move) cnt[C_NPEND] = cnt[C_NPEND2];
     cnt[C_NPEND2] = 0;
     cnt[C_TBASE] = cnt[C_NTRI];
@@ -1107,8 +1124,11 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
     while (!done && est > c.tail_threshold) {
         if (++r > 100000) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion rounds did not converge");
         unsigned ep = c.next_epoch();
-        k_round_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, r > 1);
+        // begins round r and snapshots round r-1 for the host
+        int* snap = r > 1 ? c.snap_dst((r - 1) & 1) : nullptr;
+        k_round_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, r > 1, snap);
         GCDT_LAUNCH_CHECK();
+        if (snap) c.snapshot_mark((r - 1) & 1);
         std::swap(W, W2);
         with_items(c.ins_items, [&](auto it) {
             constexpr int IT = decltype(it)::value;
@@ -1129,7 +1149,6 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
                 s, P, c.dcnt.p + C_NPEND, P2, c.dcnt.p + C_NPEND2, ep + 1, W2, c.dcnt.p + C_NW2, (int)wcap);
             GCDT_LAUNCH_CHECK();
         });
-        c.snapshot(r & 1);
         std::swap(P, P2);
         if (r >= 2) {
             const int* h = c.wait_snapshot((r - 1) & 1);  // round r-1 (round r is queued)
@@ -1139,6 +1158,7 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
         }
     }
     if (r >= 1) {
+        c.snapshot(r & 1);
         const int* h = c.wait_snapshot(r & 1);
         check_round(h);
         if (h[C_NPEND2] == 0) done = true;
@@ -1146,7 +1166,7 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
     }
     if (!done) {
         // finish in one persistent workgroup; its input is P with the count moved
-        k_round_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, r > 0);
+        k_round_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, r > 0, nullptr);
         GCDT_LAUNCH_CHECK();
         unsigned ep0 = c.epoch + 1;
         {
@@ -1170,8 +1190,13 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
 // ================================================================== flip rounds
 
 // Flip round prologue: the previous round's output list becomes the input.
-__global__ void k_flip_begin(int* cnt, int move)
+// With snap set, it first copies the previous round's counters to the host
+// ring.  C_NUM threads.
+__global__ void k_flip_begin(int* cnt, int move, int* snap)
 {
+    if (snap) snap[threadIdx.x] = cnt[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     if (move) cnt[C_NACT] = cnt[C_NACT2];
     cnt[C_NACT2] = 0;
     cnt[C_NFLIP] = 0;
@@ -1468,8 +1493,11 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         if (++r > 1000000) throw Error(GCDT_ERR_NOT_CONVERGED, "flip rounds did not converge");
         unsigned ep = c.next_epoch();
         int cnt_idx = (r == 1 && !init) ? C_CONST : C_NACT;
-        k_flip_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, r > 1);
+        // begins round r and snapshots round r-1 for the host
+        int* snap = r > 1 ? c.snap_dst((r - 1) & 1) : nullptr;
+        k_flip_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, r > 1, snap);
         GCDT_LAUNCH_CHECK();
+        if (snap) c.snapshot_mark((r - 1) & 1);
         with_items(c.flip_items, [&](auto it) {
             constexpr int IT = decltype(it)::value;
             {
@@ -1490,7 +1518,6 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
             k_flip_fix<E><<<grid_for(est), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, ep);
             GCDT_LAUNCH_CHECK();
         }
-        c.snapshot(r & 1);
         act = nxt;
         std::swap(nxt, other);
         if (r >= 2) {
@@ -1501,6 +1528,7 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         }
     }
     if (r >= 1) {
+        c.snapshot(r & 1);
         const int* h = c.wait_snapshot(r & 1);
         check_round(h);
         if (h[C_NACT2] == 0) done = true;
@@ -1527,7 +1555,7 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
             nxt = la.p;
             c.set(C_NACT, est);
         } else if (r >= 1) {
-            k_flip_begin<<<1, 1, 0, c.st>>>(c.dcnt.p, 1);
+            k_flip_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, 1, nullptr);
             GCDT_LAUNCH_CHECK();
         }
         {

@@ -147,9 +147,14 @@ struct Ctx {
     void set(int i, int v);
     // pipelined snapshots of all counters: ring of 2 pinned slots + events
     int* hring = nullptr;
+    int* hring_dev = nullptr;  // the same pinned ring, as the device writes it
     hipEvent_t ring_ev[2] = {nullptr, nullptr};
     void snapshot(int slot);          // async copy of the counters + event
     const int* wait_snapshot(int slot);  // waits for that copy
+    // destination for a round-begin kernel that copies the counters itself;
+    // snapshot_mark(slot) then records the event after that launch
+    int* snap_dst(int slot) const { return hring_dev + C_NUM * slot; }
+    void snapshot_mark(int slot);
     void zero_counters();
     void check_err();  // throws on a device error flag
     // deferred pool releases of a solve on a non-null stream
