@@ -1203,6 +1203,54 @@ __global__ void k_flip_begin(int* cnt, int move, int* snap)
     cnt[C_NPROP] = 0;
 }
 
+// Claims of the short rounds of one persistent workgroup, in an LDS hash
+// table instead of the records' claim words (device-scope atomics and
+// atomic loads there cost a trip to the memory side each; these rounds are
+// chains of such trips).  Open addressing on the triangle id.  Tags (-1) and
+// keys (0) of a table are cleared before the round that uses it.
+constexpr int LCL_CAP = 2048;      // entries per table
+constexpr int LCL_MAX_LIST = 512;  // rounds of at most this many items use it (2 claims per item)
+
+struct LdsClaims {
+    int* tag;
+    unsigned long long* key;
+    __device__ void max(const FlipP& f, int t, unsigned long long k) const
+    {
+        unsigned h = hash32((uint32_t)t) & (LCL_CAP - 1);
+        for (int probe = 0; probe < LCL_CAP; probe++) {
+            int g = tag[h];
+            if (g == -1) g = atomicCAS(&tag[h], -1, t);
+            if (g == -1 || g == t) {
+                atomicMax(&key[h], k);
+                return;
+            }
+            h = (h + 1) & (LCL_CAP - 1);
+        }
+        atomicOr(f.err, MERR_CAPACITY);
+    }
+    __device__ unsigned long long find(int t) const
+    {
+        unsigned h = hash32((uint32_t)t) & (LCL_CAP - 1);
+        for (int probe = 0; probe < LCL_CAP; probe++) {
+            int g = tag[h];
+            if (g == t) return key[h];
+            if (g == -1) return 0ull;
+            h = (h + 1) & (LCL_CAP - 1);
+        }
+        return 0ull;
+    }
+    template <bool COH>
+    __device__ unsigned long long get(const FlipP&, int t, unsigned long long) const
+    {
+        return find(t);
+    }
+    template <bool COH>
+    __device__ unsigned long long load(const FlipP&, int t) const
+    {
+        return find(t);
+    }
+};
+
 // Flip engines: the array-per-field layout (short lists on a mesh whose
 // arrays stay in place) and the packed 64-byte records (whole-mesh rounds).
 // Both keep the same claims protocol; apply() sets mark = epoch + 1 on the
@@ -1212,6 +1260,9 @@ struct EngSoA {
     Flip f;
     __device__ bool test(int t, unsigned ep, bool all, bool single) const { return flip_test(f, t, ep, all, single); }
     __device__ bool fixtest(int t, unsigned ep, bool all, bool single) const { return test(t, ep, all, single); }
+    // no LDS claims for this layout (the persistent tail never asks for them)
+    __device__ bool fixtest_l(int t, unsigned ep, const LdsClaims&) const { return test(t, ep, false, false); }
+    __device__ int apply_l(int t, unsigned ep, int* nn, const LdsClaims&) const { return apply<true>(t, ep, nn); }
     template <bool COH>
     __device__ int apply(int t, unsigned ep, int* nn) const
     {
@@ -1237,6 +1288,14 @@ struct EngPacked {
     __device__ bool fixtest(int t, unsigned ep, bool all, bool single) const
     {
         return flip_fixtest_p(f, t, ep, all, single);
+    }
+    __device__ bool fixtest_l(int t, unsigned ep, const LdsClaims& cl) const
+    {
+        return flip_fixtest_p(f, t, ep, false, false, cl);
+    }
+    __device__ int apply_l(int t, unsigned ep, int* nn, const LdsClaims& cl) const
+    {
+        return flip_apply_p<true>(f, t, ep, nn, cl);
     }
     template <bool COH>
     __device__ int apply(int t, unsigned ep, int* nn) const
@@ -1342,6 +1401,10 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int*
     // after the round's first barrier, which every thread passes only after
     // reading the pair two rounds back
     __shared__ int s_n2[2], s_nf[2];
+    // claims of rounds with at most LCL_MAX_LIST items (fused engines), one
+    // table per round parity: the other one is cleared while a round applies
+    __shared__ int s_tag[2][LCL_CAP];
+    __shared__ unsigned long long s_key[2][LCL_CAP];
     int n = cnt[C_NACT];
     unsigned ep = epoch0;
     int r = 0, err = 0, flips = 0;
@@ -1350,12 +1413,20 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int*
         s_n2[0] = 0;
         s_nf[0] = 0;
     }
+    // LDS starts with whatever the CU held before: clear tags and keys
+    for (int i = threadIdx.x; i < 2 * LCL_CAP; i += TAIL_BS) {
+        (&s_tag[0][0])[i] = -1;
+        (&s_key[0][0])[i] = 0ull;
+    }
     __syncthreads();
     while (n > 0 && r < max_rounds) {
         const int q = r & 1;
+        const bool lds = E::kFused && n <= LCL_MAX_LIST;
+        const LdsClaims cl{s_tag[q], s_key[q]};
         // fused engines settle the previous round's flips while testing
         for (int i = threadIdx.x; i < n; i += TAIL_BS) {
-            if (E::kFused) e.fixtest(A[i], ep, false, false);
+            if (lds) e.fixtest_l(A[i], ep, cl);
+            else if (E::kFused) e.fixtest(A[i], ep, false, false);
             else e.test(A[i], ep, false, false);
         }
         __syncthreads();
@@ -1363,9 +1434,13 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int*
             s_n2[q ^ 1] = 0;
             s_nf[q ^ 1] = 0;
         }
+        for (int i = threadIdx.x; i < LCL_CAP; i += TAIL_BS) {
+            s_tag[q ^ 1][i] = -1;
+            s_key[q ^ 1][i] = 0ull;
+        }
         for (int i = threadIdx.x; i < n; i += TAIL_BS) {
             int t = A[i], nn = -1;
-            int res = e.template apply<true>(t, ep, &nn);
+            int res = lds ? e.apply_l(t, ep, &nn, cl) : e.template apply<true>(t, ep, &nn);
             if (res == 1) {
                 B[atomicAdd(&s_n2[q], 1)] = t;
                 e.set_mark(t, (int)ep + 1);

@@ -510,6 +510,25 @@ __host__ __device__ inline bool flip_test_p(const FlipP& f, int t, unsigned epoc
     return cand != 3;
 }
 
+// Where a packed flip round keeps its claims: the claim word of each record
+// (RecClaims), or a table in LDS for the short rounds of one persistent
+// workgroup (mesh.hip).  get<COH> returns t's claim; `loaded` is the word
+// already read with t's record, used when no re-read is needed.
+struct RecClaims {
+    __host__ __device__ void max(const FlipP& f, int t, unsigned long long key) const { at_max(&f.r[t].claim, key); }
+    template <bool COH>
+    __host__ __device__ unsigned long long get(const FlipP& f, int t, unsigned long long loaded) const
+    {
+        return COH ? ld_claim<COH>(&f.r[t].claim) : loaded;
+    }
+    // t's claim when its record has not been read
+    template <bool COH>
+    __host__ __device__ unsigned long long load(const FlipP& f, int t) const
+    {
+        return ld_claim<COH>(&f.r[t].claim);
+    }
+};
+
 // flip_fix of the previous round and flip_test of this one in a single pass
 // over t's record and its neighbours' (each neighbour record is loaded once
 // for both).  Runs over the round's list, which holds every triangle flipped
@@ -525,8 +544,9 @@ __host__ __device__ inline bool flip_test_p(const FlipP& f, int t, unsigned epoc
 //   vertices is resolved to its partner of that flip.
 // By the end of the launch every entry and back-pointer is settled, as after
 // a separate fix launch; the last round's flips are settled by flip_fix_p.
+template <class CL = RecClaims>
 __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned epoch, bool all_active,
-                                               bool single = true)
+                                               bool single = true, const CL& cl = CL())
 {
     const FRec r = rec_load(&f.r[t]);
     const int prev = (int)epoch - 1;
@@ -585,8 +605,8 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
         if (incircle(r.p[0], r.p[1], r.p[2], rec_pt(rn, j)) > 0) {
             cand = k;
             const unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
-            at_max(&f.r[t].claim, key);
-            at_max(&f.r[n].claim, key);
+            cl.max(f, t, key);
+            cl.max(f, n, key);
         }
     }
     // most tested triangles propose nothing and already say so: a store would
@@ -603,8 +623,8 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
 // queued).  The two outer slots of each keep the old neighbours' ids; the
 // stamp marks them for settling by the next round's test (flip_fixtest_p) or
 // by flip_fix_p.
-template <bool COH = true>
-__host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoch, int* nout)
+template <bool COH = true, class CL = RecClaims>
+__host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoch, int* nout, const CL& cl = CL())
 {
     // whole records first: the proposal and both claim words lie in the two
     // lines the flip needs anyway (inside a persistent workgroup the claims
@@ -615,12 +635,12 @@ __host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoc
     const int n = get3(r.a, k);
     const FRec rn = rec_load(&f.r[n]);
     const unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
-    const unsigned long long ct = COH ? ld_claim<COH>(&f.r[t].claim) : r.claim;
-    const unsigned long long cn = COH ? ld_claim<COH>(&f.r[n].claim) : rn.claim;
+    const unsigned long long ct = cl.template get<COH>(f, t, r.claim);
+    const unsigned long long cn = cl.template get<COH>(f, n, rn.claim);
     if (ct != key || cn != key) {
         if (ct != key) {
             int t2 = (int)((ct & 0xffffffffu) / 3);
-            if ((ct >> 32) == epoch && ld_claim<COH>(&f.r[t2].claim) == ct) return 3;
+            if ((ct >> 32) == epoch && cl.template load<COH>(f, t2) == ct) return 3;
         }
         return 1;
     }
