@@ -339,16 +339,19 @@ static void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, i
     loc.release();
     pend.release();
     pend2.release();
-    int T = filter_leaves_morton(c, pts, tB.p, aB.p, sB.p, ntB, tris_out, adj_out, 2LL * N + 2);
+    // leaves in Morton order, written straight into the packed flip records
+    DBuf<FRec> rec(2LL * N + 2);
+    int T = filter_leaves_morton_rec(c, pts, tB.p, aB.p, sB.p, ntB, rec.p, 2LL * N + 2);
     tB.release();
     aB.release();
     sB.release();
     if (T != 2 * N + 2) throw Error(GCDT_ERR_INTERNAL, "leaf count " + std::to_string(T) + " != 2N+2");
     st.ms_insert = t3.ms();
-    // ---- 4. Delaunay flips
+    // ---- 4. Delaunay flips (unpacked into tris_out / adj_out)
     Timer t4(c.st);
     int f2 = 0;
-    st.flips += flip_rounds(c, Mesh{pts, tris_out, adj_out, nullptr, nullptr}, T, nullptr, 0, &f2);
+    st.flips += flip_rounds_records(c, rec.p, Mesh{pts, tris_out, adj_out, nullptr, nullptr}, T, &f2);
+    rec.release();
     st.flip_rounds = fr + f2;
     st.ms_flip = t4.ms();
     // ---- 5. constraints
@@ -866,17 +869,19 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     pend2.release();
     long long Tn = 2 * NP + 2LL * K;
     DBuf<int3> tri(Tn), adj(Tn);
-    int T = filter_leaves_inst(c, pts.p, vinst.p, NP, K, tB.p, aB.p, sB.p, nt, tri.p, adj.p, Tn);
+    DBuf<FRec> rec(Tn);
+    int T = filter_leaves_inst_rec(c, pts.p, vinst.p, NP, K, tB.p, aB.p, sB.p, nt, rec.p, Tn);
     tB.release();
     aB.release();
     sB.release();
     if (T != Tn) throw Error(GCDT_ERR_INTERNAL, "batch leaf count mismatch");
     bs.ms_insert = t1.ms();
     bs.insert_rounds = ri + ri2;
-    // ---- Delaunay flips of all instances together
+    // ---- Delaunay flips of all instances together (unpacked into tri / adj)
     Timer t2(c.st);
     int fr2 = 0;
-    bs.flips += flip_rounds(c, Mesh{pts.p, tri.p, adj.p, nullptr, nullptr}, T, nullptr, 0, &fr2);
+    bs.flips += flip_rounds_records(c, rec.p, Mesh{pts.p, tri.p, adj.p, nullptr, nullptr}, T, &fr2);
+    rec.release();
     bs.flip_rounds = fr + fr2;
     bs.ms_flip = t2.ms();
     // ---- constraints

@@ -1652,6 +1652,23 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
 
 }  // namespace
 
+long long flip_rounds_records(Ctx& c, FRec* rec, const Mesh& m, int T, int* rounds)
+{
+    if (T <= 0) {
+        if (rounds) *rounds = 0;
+        return 0;
+    }
+    EngPacked e;
+    e.f = FlipP{rec, c.dcnt.p + C_ERR};
+    long long flips = flip_driver(c, e, T, nullptr, 0, rounds);
+    {
+        ProfScope ps("flip_unpack", c.st, T);
+        k_unpack_faces<<<grid_for(T), kBlock, 0, c.st>>>(rec, T, m.tri, m.adj, m.cflag);
+        GCDT_LAUNCH_CHECK();
+    }
+    return flips;
+}
+
 long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds)
 {
     if (T <= 0) {
@@ -1666,15 +1683,7 @@ long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, 
             k_pack_faces<<<grid_for(T), kBlock, 0, c.st>>>(m.tri, m.adj, m.cflag, m.pts, T, rec.p);
             GCDT_LAUNCH_CHECK();
         }
-        EngPacked e;
-        e.f = FlipP{rec.p, c.dcnt.p + C_ERR};
-        long long flips = flip_driver(c, e, T, nullptr, 0, rounds);
-        {
-            ProfScope ps("flip_unpack", c.st, T);
-            k_unpack_faces<<<grid_for(T), kBlock, 0, c.st>>>(rec.p, T, m.tri, m.adj, m.cflag);
-            GCDT_LAUNCH_CHECK();
-        }
-        return flips;
+        return flip_rounds_records(c, rec.p, m, T, rounds);
     }
     DBuf<unsigned long long> claim(T);
     fill_u64(c, claim.p, 0, T);
@@ -2004,8 +2013,31 @@ __global__ __launch_bounds__(kBlock) void k_leaf_dest(const int* sons, const int
         dest[i] = sons[i] == -1 ? start[cellof[i]] + dest[i] : -1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_leaf_move(const int3* tri, const int3* adj, const int* dest, int T, int3* tri2,
-                                                      int3* adj2, long long cap, int* err)
+// Where the moved leaves go: plain tri/adj arrays, or straight into the
+// packed flip records (coordinates gathered here, so the flips need no
+// separate pack pass over a freshly written mesh).
+struct LeafToArrays {
+    int3* tri2;
+    int3* adj2;
+    __device__ void put(long long o, int3 v, int3 a) const
+    {
+        tri2[o] = v;
+        adj2[o] = a;
+    }
+};
+
+struct LeafToRecords {
+    FRec* rec;
+    const int2* pts;
+    __device__ void put(long long o, int3 v, int3 a) const
+    {
+        rec_store(&rec[o], v, a, pts[v.x], pts[v.y], pts[v.z], FREC_NOCAND | FREC_NOPARTNER, 0, 0ull);
+    }
+};
+
+template <class OUT>
+__global__ __launch_bounds__(kBlock) void k_leaf_move(const int3* tri, const int3* adj, const int* dest, int T, OUT out,
+                                                      long long cap, int* err)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
         int o = dest[i];
@@ -2014,18 +2046,17 @@ __global__ __launch_bounds__(kBlock) void k_leaf_move(const int3* tri, const int
             atomicOr(err, MERR_CAPACITY);
             continue;
         }
-        tri2[o] = tri[i];
         int3 a = adj[i];
         a.x = a.x >= 0 ? dest[a.x] : -1;
         a.y = a.y >= 0 ? dest[a.y] : -1;
         a.z = a.z >= 0 ? dest[a.z] : -1;
-        adj2[o] = a;
+        out.put(o, tri[i], a);
     }
 }
 
-template <class KEY>
+template <class KEY, class OUT>
 static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const int2* pts, const int3* tri,
-                                const int3* adj, const int* sons, int T, int3* tri2, int3* adj2, long long cap)
+                                const int3* adj, const int* sons, int T, const OUT& out, long long cap)
 {
     DBuf<int> cnt((size_t)ncells + 1), cellof(T > 0 ? T : 1), dest(T > 0 ? T : 1);
     fill_i32(c, cnt.p, 0, ncells + 1);
@@ -2035,8 +2066,11 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
     int n = c.get(C_TOTAL);
     k_leaf_dest<<<grid_for(T), kBlock, 0, c.st>>>(sons, cellof.p, T, cnt.p, dest.p);
     GCDT_LAUNCH_CHECK();
-    k_leaf_move<<<grid_for(T), kBlock, 0, c.st>>>(tri, adj, dest.p, T, tri2, adj2, cap, c.dcnt.p + C_ERR);
-    GCDT_LAUNCH_CHECK();
+    {
+        ProfScope ps("leaf_move", c.st, T);
+        k_leaf_move<OUT><<<grid_for(T), kBlock, 0, c.st>>>(tri, adj, dest.p, T, out, cap, c.dcnt.p + C_ERR);
+        GCDT_LAUNCH_CHECK();
+    }
     c.check_err();
     return n;
 }
@@ -2044,7 +2078,13 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
 int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2,
                          int3* adj2, long long cap)
 {
-    return filter_leaves_sorted(c, LeafKeyOne{}, BITMAP_CELLS, pts, tri, adj, sons, T, tri2, adj2, cap);
+    return filter_leaves_sorted(c, LeafKeyOne{}, BITMAP_CELLS, pts, tri, adj, sons, T, LeafToArrays{tri2, adj2}, cap);
+}
+
+int filter_leaves_morton_rec(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T,
+                             FRec* rec, long long cap)
+{
+    return filter_leaves_sorted(c, LeafKeyOne{}, BITMAP_CELLS, pts, tri, adj, sons, T, LeafToRecords{rec, pts}, cap);
 }
 
 // Batch meshes: instance-major, then the Morton order of the centroid's
@@ -2062,7 +2102,15 @@ struct LeafKeyInst {
 int filter_leaves_inst(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
                        const int3* adj, const int* sons, int T, int3* tri2, int3* adj2, long long cap)
 {
-    return filter_leaves_sorted(c, LeafKeyInst{vinst, NP}, (long long)K << 8, pts, tri, adj, sons, T, tri2, adj2, cap);
+    return filter_leaves_sorted(c, LeafKeyInst{vinst, NP}, (long long)K << 8, pts, tri, adj, sons, T,
+                                LeafToArrays{tri2, adj2}, cap);
+}
+
+int filter_leaves_inst_rec(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
+                           const int3* adj, const int* sons, int T, FRec* rec, long long cap)
+{
+    return filter_leaves_sorted(c, LeafKeyInst{vinst, NP}, (long long)K << 8, pts, tri, adj, sons, T,
+                                LeafToRecords{rec, pts}, cap);
 }
 
 int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2, int3* adj2,

@@ -204,6 +204,9 @@ int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int n
                   int* rounds);
 // flip to (constrained) Delaunay; init list (nullptr = all T triangles)
 long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds);
+// whole-mesh flip rounds on records already packed (filter_leaves_*_rec);
+// the result is unpacked into m.tri / m.adj (and m.cflag when set)
+long long flip_rounds_records(Ctx& c, FRec* rec, const Mesh& m, int T, int* rounds);
 // spatial order: points sorted by the Morton code of their 1024^2 cell
 // (out[k] = pts[perm[k]], inv = perm^-1)
 // Counting sort of points by 20-bit Morton cell; cell_end[BITMAP_CELLS + 1]
@@ -249,6 +252,11 @@ int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* a
 // batch meshes: leaves sorted by (instance, 16x16 centroid cell)
 int filter_leaves_inst(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
                        const int3* adj, const int* sons, int T, int3* tri2, int3* adj2, long long cap);
+// the same two orders, written straight into packed flip records
+int filter_leaves_morton_rec(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T,
+                             FRec* rec, long long cap);
+int filter_leaves_inst_rec(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
+                           const int3* adj, const int* sons, int T, FRec* rec, long long cap);
 // bitmap CSR over 1024^2 cells; allocates offsets/tris from the pool
 void build_bitmap(Ctx& c, const int2* pts, const int3* tri, int T, int** offsets, int** tris, int* ntris);
 // mark vertices of tri[0..T) in isv (must be zeroed), then locate the other
