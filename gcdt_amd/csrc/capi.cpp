@@ -868,12 +868,12 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     pend.release();
     pend2.release();
     long long Tn = 2 * NP + 2LL * K;
-    DBuf<int3> tri(Tn), adj(Tn);
     DBuf<FRec> rec(Tn);
     int T = filter_leaves_inst_rec(c, pts.p, vinst.p, NP, K, tB.p, aB.p, sB.p, nt, rec.p, Tn);
     tB.release();
     aB.release();
     sB.release();
+    DBuf<int3> tri(Tn), adj(Tn);  // after the insertion DAG is gone: a lower peak
     if (T != Tn) throw Error(GCDT_ERR_INTERNAL, "batch leaf count mismatch");
     bs.ms_insert = t1.ms();
     bs.insert_rounds = ri + ri2;

@@ -27,10 +27,13 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-# bytes of device memory per point during a solve (insertion DAG 4N slots x
-# 36 B, per-point state 24 B, final mesh 2N x 24 B, flip/constraint state
-# 2N x 32 B) rounded up; per request overhead for the bitmap and scratch
-BYTES_PER_POINT = 320
+# bytes of device memory per point during a solve: the insertion DAG (4N
+# slots x 28 B) alive while the leaves are written into the packed flip
+# records (2N x 64 B), then records + final mesh (2N x 24 B) + flip lists, plus
+# per-point state and the sorted copy; a 4,096-instance batched arena peaks
+# at about 364 B per point, rounded up here; per request overhead for the
+# bitmap and scratch
+BYTES_PER_POINT = 384
 BYTES_PER_REQUEST = 16 << 20
 
 
