@@ -2005,12 +2005,16 @@ __global__ __launch_bounds__(kBlock) void k_leaf_cells(KEY kf, const int2* pts, 
     }
 }
 
-// dest[] holds the ranks on entry and the destinations on exit.
+// dest[] holds the ranks on entry and the destinations on exit; src[] gets
+// the inverse (the leaf that lands at each destination).
 __global__ __launch_bounds__(kBlock) void k_leaf_dest(const int* sons, const int* cellof, int T, const int* start,
-                                                      int* dest)
+                                                      int* dest, int* src, long long cap)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x)
-        dest[i] = sons[i] == -1 ? start[cellof[i]] + dest[i] : -1;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        int d = sons[i] == -1 ? start[cellof[i]] + dest[i] : -1;
+        dest[i] = d;
+        if (d >= 0 && d < cap) src[d] = (int)i;
+    }
 }
 
 // Where the moved leaves go: plain tri/adj arrays, or straight into the
@@ -2035,17 +2039,14 @@ struct LeafToRecords {
     }
 };
 
+// Walks the destinations in order and gathers each one's leaf, so the
+// (64-byte record) writes are contiguous and the reads do the scattering.
 template <class OUT>
-__global__ __launch_bounds__(kBlock) void k_leaf_move(const int3* tri, const int3* adj, const int* dest, int T, OUT out,
-                                                      long long cap, int* err)
+__global__ __launch_bounds__(kBlock) void k_leaf_move(const int3* tri, const int3* adj, const int* dest,
+                                                      const int* src, int n, OUT out)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
-        int o = dest[i];
-        if (o < 0) continue;
-        if (o >= cap) {
-            atomicOr(err, MERR_CAPACITY);
-            continue;
-        }
+    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x) {
+        int i = src[o];
         int3 a = adj[i];
         a.x = a.x >= 0 ? dest[a.x] : -1;
         a.y = a.y >= 0 ? dest[a.y] : -1;
@@ -2064,14 +2065,15 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
     GCDT_LAUNCH_CHECK();
     scan_excl(c, cnt.p, cnt.p, ncells);
     int n = c.get(C_TOTAL);
-    k_leaf_dest<<<grid_for(T), kBlock, 0, c.st>>>(sons, cellof.p, T, cnt.p, dest.p);
+    if (n > cap) throw Error(GCDT_ERR_CAPACITY, "more leaves than the output holds");
+    DBuf<int> src(n > 0 ? n : 1);
+    k_leaf_dest<<<grid_for(T), kBlock, 0, c.st>>>(sons, cellof.p, T, cnt.p, dest.p, src.p, cap);
     GCDT_LAUNCH_CHECK();
-    {
-        ProfScope ps("leaf_move", c.st, T);
-        k_leaf_move<OUT><<<grid_for(T), kBlock, 0, c.st>>>(tri, adj, dest.p, T, out, cap, c.dcnt.p + C_ERR);
+    if (n > 0) {
+        ProfScope ps("leaf_move", c.st, n);
+        k_leaf_move<OUT><<<grid_for(n), kBlock, 0, c.st>>>(tri, adj, dest.p, src.p, n, out);
         GCDT_LAUNCH_CHECK();
     }
-    c.check_err();
     return n;
 }
 

@@ -31,7 +31,7 @@ import numpy as np
 # slots x 28 B) alive while the leaves are written into the packed flip
 # records (2N x 64 B), then records + final mesh (2N x 24 B) + flip lists, plus
 # per-point state and the sorted copy; a 4,096-instance batched arena peaks
-# at about 364 B per point, rounded up here; per request overhead for the
+# at about 378 B per point, rounded up here; per request overhead for the
 # bitmap and scratch
 BYTES_PER_POINT = 384
 BYTES_PER_REQUEST = 16 << 20
