@@ -104,8 +104,8 @@ def main(argv=None):
                     help="requests up to this many points are merged into batched solves")
     args = ap.parse_args(argv)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: every GPU box here has one MI355X, and this CDT path has no "
-              "multi-GPU code (the sharding design is documented, not built). Run with --gpus 1.",
+        print("bench.py measures one MI355X only: this CDT path has no multi-GPU code (the sharding design "
+              "is documented, not built). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

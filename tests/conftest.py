@@ -7,5 +7,5 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box only)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X")
     config.addinivalue_line("markers", "slow: larger CPU cases")
