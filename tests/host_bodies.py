@@ -1,0 +1,48 @@
+"""Builds and binds tests/host_bodies.cpp: the solve's per-item kernel bodies
+executed on the host in seeded shuffled orders (test infrastructure)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "host_bodies.cpp")
+LIB = os.path.join(HERE, "libhost_bodies.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+def build() -> str:
+    cmd = [HIPCC, "-O2", "-fPIC", "-shared", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-o", LIB, SRC]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {SRC}:\n{r.stdout}\n{r.stderr}")
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        _lib.host_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_uint, ctypes.c_int]
+        _lib.host_solve.restype = ctypes.c_int
+    return _lib
+
+
+def solve(g: np.ndarray, cons: np.ndarray, seed: int = 0, fused: bool = True):
+    """(status, triangles (2n+2, 3), stats[8])."""
+    g = np.ascontiguousarray(g, np.int32)
+    c = np.ascontiguousarray(cons, np.int32).reshape(-1, 2)
+    n = len(g)
+    out = np.zeros((2 * n + 2, 3), np.int32)
+    st = np.zeros(8, np.int64)
+    rc = lib().host_solve(g.ctypes.data, n, c.ctypes.data, len(c), out.ctypes.data, st.ctypes.data, seed,
+                          1 if fused else 0)
+    return rc, out, st
