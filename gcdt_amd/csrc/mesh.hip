@@ -2,6 +2,7 @@
 // bitmap ("boostmap") location, parallel insertion rounds, flip rounds,
 // leaf compaction and adjacency construction.
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
@@ -33,15 +34,14 @@ thread_local std::vector<PendingFree>* g_defer = nullptr;
 int g_grid_cap = kMaxGrid;
 
 namespace {
-// Pinned counter mirrors and ring events, reused across solves: allocating
+// Pinned counter mirrors and snapshot rings, reused across solves: allocating
 // and freeing pinned memory per call costs more than a small solve and
 // hipHostFree waits for the whole device, which would serialise solves that
 // run on different streams.
 struct HostSlot {
     int* hcnt;
-    int* hring;
-    int* hring_dev;
-    hipEvent_t ev[2];
+    unsigned long long* hring;
+    unsigned long long* hring_dev;
 };
 std::mutex g_host_mu;
 std::vector<HostSlot> g_host_free;
@@ -59,11 +59,11 @@ HostSlot take_host_slot()
     HostSlot h{};
     GCDT_HIP(hipHostMalloc((void**)&h.hcnt, sizeof(int) * C_NUM, hipHostMallocDefault));
     // written by kernels (round-begin snapshots): coherent, so the writes reach
-    // host memory without a cache flush and are seen once the event completes
-    GCDT_HIP(hipHostMalloc((void**)&h.hring, sizeof(int) * 2 * C_NUM, hipHostMallocMapped | hipHostMallocCoherent));
+    // host memory without a cache flush; sequence number 0 is never issued
+    GCDT_HIP(hipHostMalloc((void**)&h.hring, sizeof(unsigned long long) * 2 * C_NUM,
+                           hipHostMallocMapped | hipHostMallocCoherent));
+    memset(h.hring, 0, sizeof(unsigned long long) * 2 * C_NUM);
     GCDT_HIP(hipHostGetDevicePointer((void**)&h.hring_dev, h.hring, 0));
-    GCDT_HIP(hipEventCreateWithFlags(&h.ev[0], hipEventDisableTiming));
-    GCDT_HIP(hipEventCreateWithFlags(&h.ev[1], hipEventDisableTiming));
     return h;
 }
 }  // namespace
@@ -80,8 +80,6 @@ Ctx::Ctx(hipStream_t s) : st(s)
     hcnt = h.hcnt;
     hring = h.hring;
     hring_dev = h.hring_dev;
-    ring_ev[0] = h.ev[0];
-    ring_ev[1] = h.ev[1];
     if (st) {
         prev_defer = g_defer;
         g_defer = &deferred;
@@ -100,27 +98,72 @@ Ctx::~Ctx()
     }
     if (hcnt) {
         std::lock_guard<std::mutex> g(g_host_mu);
-        g_host_free.push_back(HostSlot{hcnt, hring, hring_dev, {ring_ev[0], ring_ev[1]}});
+        g_host_free.push_back(HostSlot{hcnt, hring, hring_dev});
     }
 }
 
+// One counter into a snapshot slot: one 64-bit store at system scope, so the
+// host never sees a value without its sequence number.
+__device__ inline void snap_store(unsigned long long* dst, unsigned seq, int v)
+{
+    __hip_atomic_store(dst, ((unsigned long long)seq << 32) | (unsigned)v, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // the counters straight into the mapped pinned ring (a kernel costs less
-// than a copy-engine transfer of 128 bytes)
-__global__ void k_snap(const int* cnt, int* dst) { dst[threadIdx.x] = cnt[threadIdx.x]; }
+// than a copy-engine transfer)
+__global__ void k_snap(const int* cnt, unsigned long long* dst, unsigned seq)
+{
+    snap_store(&dst[threadIdx.x], seq, cnt[threadIdx.x]);
+}
+
+namespace {
+// process-wide, so a slot reused by another context never matches old words
+std::atomic<unsigned> g_snap_seq{0};
+}  // namespace
+
+unsigned Ctx::snap_begin(int slot)
+{
+    unsigned s = ++g_snap_seq;
+    if (s == 0) s = ++g_snap_seq;
+    ring_seq[slot] = s;
+    return s;
+}
 
 void Ctx::snapshot(int slot)
 {
-    k_snap<<<1, C_NUM, 0, st>>>(dcnt.p, snap_dst(slot));
+    unsigned seq = snap_begin(slot);
+    k_snap<<<1, C_NUM, 0, st>>>(dcnt.p, snap_dst(slot), seq);
     GCDT_LAUNCH_CHECK();
-    snapshot_mark(slot);
 }
-
-void Ctx::snapshot_mark(int slot) { GCDT_HIP(hipEventRecord(ring_ev[slot], st)); }
 
 const int* Ctx::wait_snapshot(int slot)
 {
-    GCDT_HIP(hipEventSynchronize(ring_ev[slot]));
-    return hring + C_NUM * slot;
+    const unsigned long long* p = hring + C_NUM * slot;
+    const unsigned want = ring_seq[slot];
+    int* out = hsnap[slot];
+    auto complete = [&]() {
+        for (int i = 0; i < C_NUM; i++) {
+            unsigned long long v = __atomic_load_n(p + i, __ATOMIC_ACQUIRE);
+            if ((unsigned)(v >> 32) != want) return false;
+            out[i] = (int)(unsigned)v;
+        }
+        return true;
+    };
+    for (unsigned spin = 1;; ++spin) {
+        if (complete()) return out;
+        if ((spin & 1023) == 0) {
+            // the words can be missing only while their kernel is queued or
+            // running: a device error or a drained stream without them ends
+            // the wait
+            hipError_t e = hipStreamQuery(st);
+            if (e == hipSuccess) {
+                if (complete()) return out;
+                throw Error(GCDT_ERR_INTERNAL, "counter snapshot missing");
+            }
+            if (e != hipErrorNotReady) GCDT_HIP(e);
+        }
+    }
 }
 
 void Ctx::read(int i, int k)
@@ -842,9 +885,9 @@ int locate_bitmap_batch(Ctx& c, const int2* pts, const int* vinst, long long NP,
 // input (count C_NPEND2 -> C_NPEND) and the slot base is recorded.
 // With snap set, it first copies the previous round's counters to the host
 // ring (one launch instead of a begin kernel plus a copy).  C_NUM threads.
-__global__ void k_round_begin(int* cnt, int move, int* snap)
+__global__ void k_round_begin(int* cnt, int move, unsigned long long* snap, unsigned seq)
 {
-    if (snap) snap[threadIdx.x] = cnt[threadIdx.x];
+    if (snap) snap_store(&snap[threadIdx.x], seq, cnt[threadIdx.x]);
     __syncthreads();
     if (threadIdx.x != 0) return;
     if (move) cnt[C_NPEND] = cnt[C_NPEND2];
@@ -1125,10 +1168,10 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
         if (++r > 100000) throw Error(GCDT_ERR_NOT_CONVERGED, "insertion rounds did not converge");
         unsigned ep = c.next_epoch();
         // begins round r and snapshots round r-1 for the host
-        int* snap = r > 1 ? c.snap_dst((r - 1) & 1) : nullptr;
-        k_round_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, r > 1, snap);
+        unsigned long long* snap = r > 1 ? c.snap_dst((r - 1) & 1) : nullptr;
+        unsigned seq = r > 1 ? c.snap_begin((r - 1) & 1) : 0u;
+        k_round_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, r > 1, snap, seq);
         GCDT_LAUNCH_CHECK();
-        if (snap) c.snapshot_mark((r - 1) & 1);
         std::swap(W, W2);
         with_items(c.ins_items, [&](auto it) {
             constexpr int IT = decltype(it)::value;
@@ -1166,7 +1209,7 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
     }
     if (!done) {
         // finish in one persistent workgroup; its input is P with the count moved
-        k_round_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, r > 0, nullptr);
+        k_round_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, r > 0, nullptr, 0u);
         GCDT_LAUNCH_CHECK();
         unsigned ep0 = c.epoch + 1;
         {
@@ -1192,9 +1235,9 @@ int insert_rounds(Ctx& c, const Ins& s_in, DBuf<int>& pend, DBuf<int>& pend2, in
 // Flip round prologue: the previous round's output list becomes the input.
 // With snap set, it first copies the previous round's counters to the host
 // ring.  C_NUM threads.
-__global__ void k_flip_begin(int* cnt, int move, int* snap)
+__global__ void k_flip_begin(int* cnt, int move, unsigned long long* snap, unsigned seq)
 {
-    if (snap) snap[threadIdx.x] = cnt[threadIdx.x];
+    if (snap) snap_store(&snap[threadIdx.x], seq, cnt[threadIdx.x]);
     __syncthreads();
     if (threadIdx.x != 0) return;
     if (move) cnt[C_NACT] = cnt[C_NACT2];
@@ -1569,10 +1612,10 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
         unsigned ep = c.next_epoch();
         int cnt_idx = (r == 1 && !init) ? C_CONST : C_NACT;
         // begins round r and snapshots round r-1 for the host
-        int* snap = r > 1 ? c.snap_dst((r - 1) & 1) : nullptr;
-        k_flip_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, r > 1, snap);
+        unsigned long long* snap = r > 1 ? c.snap_dst((r - 1) & 1) : nullptr;
+        unsigned seq = r > 1 ? c.snap_begin((r - 1) & 1) : 0u;
+        k_flip_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, r > 1, snap, seq);
         GCDT_LAUNCH_CHECK();
-        if (snap) c.snapshot_mark((r - 1) & 1);
         with_items(c.flip_items, [&](auto it) {
             constexpr int IT = decltype(it)::value;
             {
@@ -1630,7 +1673,7 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
             nxt = la.p;
             c.set(C_NACT, est);
         } else if (r >= 1) {
-            k_flip_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, 1, nullptr);
+            k_flip_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, 1, nullptr, 0u);
             GCDT_LAUNCH_CHECK();
         }
         {

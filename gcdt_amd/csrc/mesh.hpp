@@ -145,16 +145,22 @@ struct Ctx {
         return hcnt[i];
     }
     void set(int i, int v);
-    // pipelined snapshots of all counters: ring of 2 pinned slots + events
-    int* hring = nullptr;
-    int* hring_dev = nullptr;  // the same pinned ring, as the device writes it
-    hipEvent_t ring_ev[2] = {nullptr, nullptr};
-    void snapshot(int slot);          // async copy of the counters + event
-    const int* wait_snapshot(int slot);  // waits for that copy
-    // destination for a round-begin kernel that copies the counters itself;
-    // snapshot_mark(slot) then records the event after that launch
-    int* snap_dst(int slot) const { return hring_dev + C_NUM * slot; }
-    void snapshot_mark(int slot);
+    // pipelined snapshots of all counters: a ring of 2 slots in mapped pinned
+    // memory, each counter one 64-bit word (sequence number << 32 | value)
+    // that a kernel writes with one system-scope store; the host polls a slot
+    // until every word carries the slot's current sequence number (no event
+    // per round: an event marker between two kernels costs a gap of about
+    // 10 us on the device)
+    unsigned long long* hring = nullptr;
+    unsigned long long* hring_dev = nullptr;  // the same ring, as the device writes it
+    unsigned ring_seq[2] = {0, 0};
+    int hsnap[2][C_NUM];
+    void snapshot(int slot);             // a copy kernel into the slot
+    const int* wait_snapshot(int slot);  // polls until the slot is complete
+    // a kernel that snapshots the counters itself writes snap_dst(slot) with
+    // the sequence number snap_begin(slot) returns
+    unsigned long long* snap_dst(int slot) const { return hring_dev + C_NUM * slot; }
+    unsigned snap_begin(int slot);
     void zero_counters();
     void check_err();  // throws on a device error flag
     // deferred pool releases of a solve on a non-null stream
