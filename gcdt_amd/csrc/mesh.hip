@@ -1352,13 +1352,15 @@ struct EngPacked {
 // Tests the listed triangles; the ones that propose a flip are appended to
 // `prop`, the only list the apply step reads.
 template <class E, int FT_ITEMS>
-__global__ __launch_bounds__(kBlock) void k_flip_test(E e, const int* list, const int* cnt, unsigned epoch, int single,
-                                                      int* prop, int* nprop)
+__global__ __launch_bounds__(kBlock) void k_flip_test(E e, const int* list, const int* cnt, int lcap, unsigned epoch,
+                                                      int single, int* prop, int* nprop, int cap, int* eflag)
 {
     __shared__ int s_buf[STAGE_CAP];
     __shared__ int s_base;
-    Staged<STAGE_CAP> sp(s_buf, &s_base, prop, nprop);
-    int n = *cnt;
+    // a triangle proposes at most once per round, so `prop` (cap entries)
+    // cannot overflow; a miscount flags MERR_CAPACITY instead of writing past it
+    Staged<STAGE_CAP> sp(s_buf, &s_base, prop, nprop, cap, eflag, MERR_CAPACITY);
+    int n = min(*cnt, lcap);
     TILE_LOOP(base, n, kBlock * FT_ITEMS)
     {
         int q[FT_ITEMS];
@@ -1385,12 +1387,16 @@ __global__ __launch_bounds__(kBlock) void k_flip_test(E e, const int* list, cons
 
 template <class E, int FA_ITEMS>
 __global__ __launch_bounds__(kBlock) void k_flip_apply(E e, const int* list, const int* cnt, unsigned epoch, int* list2,
-                                                       int* cnt2, int* flipped, int* nflip)
+                                                       int* cnt2, int* flipped, int* nflip, int cap, int* eflag)
 {
     __shared__ int s_buf[STAGE_CAP], s_buf2[STAGE_CAP];
     __shared__ int s_base, s_base2;
-    Staged<STAGE_CAP> sq(s_buf, &s_base, list2, cnt2), sf(s_buf2, &s_base2, flipped, nflip);
-    int n = *cnt;
+    // each triangle enters the next list at most once per round (a lost
+    // proposer that keeps its own claim is flipped by no one), so both lists
+    // hold at most cap entries; a miscount flags MERR_CAPACITY
+    Staged<STAGE_CAP> sq(s_buf, &s_base, list2, cnt2, cap, eflag, MERR_CAPACITY),
+        sf(s_buf2, &s_base2, flipped, nflip, cap, eflag, MERR_CAPACITY);
+    int n = min(*cnt, cap);
     TILE_LOOP(base, n, kBlock * FA_ITEMS)
     {
         int qa[FA_ITEMS], qb[FA_ITEMS];
@@ -1428,17 +1434,17 @@ __global__ __launch_bounds__(kBlock) void k_flip_apply(E e, const int* list, con
 }
 
 template <class E>
-__global__ __launch_bounds__(kBlock) void k_flip_fix(E e, const int* list, const int* cnt, unsigned epoch)
+__global__ __launch_bounds__(kBlock) void k_flip_fix(E e, const int* list, const int* cnt, int cap, unsigned epoch)
 {
-    int n = *cnt;
+    int n = min(*cnt, cap);
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         e.fix(list[i], epoch);
 }
 
 // Persistent single-workgroup tail of the flip rounds.
 template <class E>
-__global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int* lf, int* cnt, unsigned epoch0,
-                                                       int max_rounds)
+__global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int* lf, int* cnt, int cap,
+                                                       unsigned epoch0, int max_rounds)
 {
     // round counters double-buffered by round parity: a round's pair is reset
     // after the round's first barrier, which every thread passes only after
@@ -1448,7 +1454,7 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int*
     // table per round parity: the other one is cleared while a round applies
     __shared__ int s_tag[2][LCL_CAP];
     __shared__ unsigned long long s_key[2][LCL_CAP];
-    int n = cnt[C_NACT];
+    int n = min(cnt[C_NACT], cap);  // A, B and lf hold cap entries each
     unsigned ep = epoch0;
     int r = 0, err = 0, flips = 0;
     int nf = 0;
@@ -1485,20 +1491,30 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int*
             int t = A[i], nn = -1;
             int res = lds ? e.apply_l(t, ep, &nn, cl) : e.template apply<true>(t, ep, &nn);
             if (res == 1) {
-                B[atomicAdd(&s_n2[q], 1)] = t;
+                int b = atomicAdd(&s_n2[q], 1);
+                if (b < cap) B[b] = t;
                 e.set_mark(t, (int)ep + 1);
             } else if (res == 2) {
                 int b = atomicAdd(&s_n2[q], 2);
-                B[b] = t;
-                B[b + 1] = nn;
+                if (b + 1 < cap) {
+                    B[b] = t;
+                    B[b + 1] = nn;
+                }
                 int w = atomicAdd(&s_nf[q], 2);
-                lf[w] = t;
-                lf[w + 1] = nn;
+                if (w + 1 < cap) {
+                    lf[w] = t;
+                    lf[w + 1] = nn;
+                }
             }
         }
         __syncthreads();
         int n2 = s_n2[q];
         nf = s_nf[q];
+        if (n2 > cap || nf > cap) {  // cannot happen (one entry per triangle): flagged, not written
+            if (threadIdx.x == 0) atomicOr(&cnt[C_ERR], MERR_CAPACITY);
+            n = 0;
+            break;
+        }
         if (!E::kFused) {
             for (int i = threadIdx.x; i < nf; i += TAIL_BS) e.fix(lf[i], ep);
             __syncthreads();
@@ -1621,19 +1637,21 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
             {
                 ProfScope ps("flip_test", c.st, 0);
                 k_flip_test<E, IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(
-                    e, act, c.dcnt.p + cnt_idx, ep, c.single_tester, lp.p, c.dcnt.p + C_NPROP);
+                    e, act, c.dcnt.p + cnt_idx, (init && act == init) ? ninit : T, ep, c.single_tester, lp.p,
+                    c.dcnt.p + C_NPROP, T, c.dcnt.p + C_ERR);
                 GCDT_LAUNCH_CHECK();
             }
             {
                 ProfScope ps("flip_apply", c.st, 0);
                 k_flip_apply<E, IT><<<grid_for((est + IT - 1) / IT), kBlock, 0, c.st>>>(
-                    e, lp.p, c.dcnt.p + C_NPROP, ep, nxt, c.dcnt.p + C_NACT2, lf.p, c.dcnt.p + C_NFLIP);
+                    e, lp.p, c.dcnt.p + C_NPROP, ep, nxt, c.dcnt.p + C_NACT2, lf.p, c.dcnt.p + C_NFLIP, T,
+                    c.dcnt.p + C_ERR);
                 GCDT_LAUNCH_CHECK();
             }
         });
         if (!E::kFused) {
             ProfScope ps("flip_fix", c.st, 0);
-            k_flip_fix<E><<<grid_for(est), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, ep);
+            k_flip_fix<E><<<grid_for(est), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, T, ep);
             GCDT_LAUNCH_CHECK();
         }
         act = nxt;
@@ -1657,7 +1675,7 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
             // settle the last round's flips (no test follows; the tail's first
             // test settles them when the tail runs)
             ProfScope ps("flip_fix", c.st, h[C_NFLIP]);
-            k_flip_fix<E><<<grid_for(h[C_NFLIP]), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, c.epoch);
+            k_flip_fix<E><<<grid_for(h[C_NFLIP]), kBlock, 0, c.st>>>(e, lf.p, c.dcnt.p + C_NFLIP, T, c.epoch);
             GCDT_LAUNCH_CHECK();
         }
     }
@@ -1672,13 +1690,19 @@ long long flip_driver(Ctx& c, const E& e, int T, const int* init, int ninit, int
             act = lb.p;
             nxt = la.p;
             c.set(C_NACT, est);
+        } else if (init && act == init) {
+            // the tail alternates its two lists: work on a copy of the
+            // caller's list (C_NACT holds ninit already)
+            GCDT_HIP(hipMemcpyAsync(lb.p, init, sizeof(int) * (size_t)ninit, hipMemcpyDeviceToDevice, c.st));
+            act = lb.p;
+            nxt = la.p;
         } else if (r >= 1) {
             k_flip_begin<<<1, C_NUM, 0, c.st>>>(c.dcnt.p, 1, nullptr, 0u);
             GCDT_LAUNCH_CHECK();
         }
         {
             ProfScope ps("flip_tail", c.st, est);
-            k_flip_tail<E><<<1, TAIL_BS, 0, c.st>>>(e, (int*)act, nxt, lf.p, c.dcnt.p, ep0, 1 << 20);
+            k_flip_tail<E><<<1, TAIL_BS, 0, c.st>>>(e, (int*)act, nxt, lf.p, c.dcnt.p, T, ep0, 1 << 20);
             GCDT_LAUNCH_CHECK();
         }
         c.read(0, C_NUM);
