@@ -39,7 +39,7 @@ extern "C" {
 #define GCDT_ERR_CROSSING 5       /* two constraint segments cross */
 #define GCDT_ERR_SCRATCH 6        /* caller scratch too small */
 #define GCDT_ERR_LOCATE 7         /* a point could not be located / duplicate point */
-#define GCDT_ERR_CAPACITY 8       /* triangle slot capacity exceeded */
+#define GCDT_ERR_CAPACITY 8       /* triangle slot or work-list capacity exceeded */
 #define GCDT_ERR_INTERNAL 9       /* mesh consistency check failed */
 
 typedef struct { int32_t x, y; } gcdt_int2;
