@@ -24,6 +24,7 @@ from __future__ import annotations
 import ctypes
 import json
 import os
+import sys
 import time
 
 from .lib import check, lib
@@ -110,6 +111,9 @@ def tune(kind: str, run, candidates=None, repeats: int = 5, key: str | None = No
         for v in values:
             trial = dict(choice, **{name: v})
             apply(trial)
+            # named before it runs, so a failing trial is identified by the
+            # last line a run printed
+            print(f"# tune {kind}: {trial}", file=sys.stderr, flush=True)
             t = _time(run, repeats)
             trials.append({**trial, "ms": round(t * 1000.0, 3)})
             times[v] = t
