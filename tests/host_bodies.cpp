@@ -32,6 +32,7 @@ static int g_err = 0;
 static unsigned g_epoch = 0;
 static unsigned g_seed = 0;  // item-order shuffles
 static int g_fused = 1;      // packed rounds settle flips inside the next test
+static int g_two = 0;        // 1: both triangles of an edge test it in every round (else only short lists)
 
 static void shuffle(std::vector<int>& v, unsigned salt)
 {
@@ -106,7 +107,7 @@ static long long flip_rounds(const Mesh& m, int T, int* rounds)
     while (!act.empty()) {
         r++;
         unsigned ep = ++g_epoch;
-        for (int t : act) flip_test(f, t, ep, r == 1, act.size() > 64);
+        for (int t : act) flip_test(f, t, ep, r == 1, !g_two && act.size() > 64);
         std::vector<int> nx, fl;
         for (int t : act) {
             int n = -1;
@@ -154,8 +155,8 @@ static long long flip_rounds_p(const Mesh& m, int T, int* rounds)
         std::vector<int> ord(act);
         shuffle(ord, ep);
         for (int t : ord) {
-            if (g_fused) flip_fixtest_p(f, t, ep, r == 1, act.size() > 64);
-            else flip_test_p(f, t, ep, r == 1, act.size() > 64);
+            if (g_fused) flip_fixtest_p(f, t, ep, r == 1, !g_two && act.size() > 64);
+            else flip_test_p(f, t, ep, r == 1, !g_two && act.size() > 64);
         }
         std::vector<int> nx, fl;
         shuffle(ord, ~ep);
@@ -230,11 +231,12 @@ static void dag_init(DAG& d, int cap, int N)
 // constraint cavities and the constrained flip pass.  Returns 0 or an error
 // code (the hundreds say which stage failed); tris_out holds 2N+2 triangles.
 extern "C" int host_solve(const int* xy, int N, const int* consp, int C, int* tris_out, long long* stats,
-                          unsigned seed, int fused)
+                          unsigned seed, int fused, int two_testers)
 {
     g_err = 0;
     g_seed = seed;
     g_fused = fused;
+    g_two = two_testers;
     std::vector<int2> pts(N + 4);
     for (int i = 0; i < N; i++) pts[i] = make_int2(xy[2 * i], xy[2 * i + 1]);
     pts[N] = make_int2(0, 0);

@@ -31,18 +31,20 @@ def lib():
             build()
         _lib = ctypes.CDLL(LIB)
         _lib.host_solve.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
-                                    ctypes.c_void_p, ctypes.c_uint, ctypes.c_int]
+                                    ctypes.c_void_p, ctypes.c_uint, ctypes.c_int, ctypes.c_int]
         _lib.host_solve.restype = ctypes.c_int
     return _lib
 
 
-def solve(g: np.ndarray, cons: np.ndarray, seed: int = 0, fused: bool = True):
-    """(status, triangles (2n+2, 3), stats[8])."""
+def solve(g: np.ndarray, cons: np.ndarray, seed: int = 0, fused: bool = True, two_testers: bool = False):
+    """(status, triangles (2n+2, 3), stats[8]).  two_testers: both triangles
+    of an edge test it in every round (otherwise only in lists of at most 64
+    triangles, as on the GPU with single_tester set)."""
     g = np.ascontiguousarray(g, np.int32)
     c = np.ascontiguousarray(cons, np.int32).reshape(-1, 2)
     n = len(g)
     out = np.zeros((2 * n + 2, 3), np.int32)
     st = np.zeros(8, np.int64)
     rc = lib().host_solve(g.ctypes.data, n, c.ctypes.data, len(c), out.ctypes.data, st.ctypes.data, seed,
-                          1 if fused else 0)
+                          1 if fused else 0, 1 if two_testers else 0)
     return rc, out, st

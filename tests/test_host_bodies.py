@@ -11,8 +11,8 @@ from oracle import oracle as O
 from tests import host_bodies as HB
 
 
-def _check(g, c, seed, fused=True):
-    rc, tris, st = HB.solve(g, c, seed=seed, fused=fused)
+def _check(g, c, seed, fused=True, two_testers=False):
+    rc, tris, st = HB.solve(g, c, seed=seed, fused=fused, two_testers=two_testers)
     assert rc == 0, (rc, seed)
     v = O.validate(g, tris, c)
     assert v["valid"], (seed, v)
@@ -57,3 +57,13 @@ def test_clamped_cluster_fans(seed):
     p, _ = gen.dedup(np.concatenate([u, cl.astype(np.int64)]))
     st = _check(gen.to_grid(p), np.zeros((0, 2), np.int32), seed)
     assert st[2] > 50  # flip rounds: the fans do form
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_two_testers_per_edge(seed):
+    """Both triangles of an edge test it in every round (the single_tester 0
+    setting): more proposals lose, and every round must still list each
+    triangle at most once (the emulator fails a case that lists one twice)."""
+    p, c = gen.uniform_cdt(n=12_000, c=120, seed_pts=500 + seed, seed_cons=600 + seed)
+    _check(gen.to_grid(p), c, seed, two_testers=True)
+    _check(gen.to_grid(p), c, seed, fused=False, two_testers=True)
