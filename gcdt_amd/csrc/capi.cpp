@@ -925,20 +925,27 @@ extern "C" int gcdt_set_tunable(const char* name, int value)
                                                                        std::to_string(hi) + "]");
     };
     int rc;
+    // only values that a GPU parity test covers are accepted
+    auto cleared = [&]() {
+        return tunable_cleared(n, value)
+                   ? GCDT_OK
+                   : fail(GCDT_ERR_INVALID, n + "=" + std::to_string(value) + " is not covered by a GPU parity test");
+    };
     if (n == "tail_threshold") {
-        if ((rc = range(0, 1 << 20))) return rc;
+        if ((rc = range(0, 1 << 20)) || (rc = cleared())) return rc;
         Tunables::get().tail_threshold = value;
     } else if (n == "pivot") {
-        if ((rc = range(0, 1))) return rc;
+        if ((rc = range(0, 1)) || (rc = cleared())) return rc;
         Tunables::get().pivot = value;
     } else if (n == "single_tester") {
-        if ((rc = range(0, 1))) return rc;
+        if ((rc = range(0, 1)) || (rc = cleared())) return rc;
         Tunables::get().single_tester = value;
     } else if (n == "grid_cap") {
-        if ((rc = range(256, 65536))) return rc;
+        if ((rc = range(256, 65536)) || (rc = cleared())) return rc;
         g_grid_cap = value;
     } else if (n == "flip_items" || n == "ins_items") {
         if (value != 1 && value != 2 && value != 4) return fail(GCDT_ERR_INVALID, n + " must be 1, 2 or 4");
+        if ((rc = cleared())) return rc;
         (n == "flip_items" ? Tunables::get().flip_items : Tunables::get().ins_items) = value;
     } else {
         return fail(GCDT_ERR_INVALID, "unknown tunable " + n);

@@ -17,14 +17,36 @@ namespace gcdt {
 
 // ================================================================== context
 
+// Values of each tunable that a GPU parity test covers.  Anything else is
+// refused, by gcdt_set_tunable and by the environment overrides alike.
+bool tunable_cleared(const std::string& name, int v)
+{
+    if (name == "tail_threshold") return v == 0 || v == 4096 || v == (1 << 20);
+    if (name == "grid_cap") return v == kMaxGrid;
+    if (name == "flip_items" || name == "ins_items") return v == 1;
+    if (name == "pivot" || name == "single_tester") return v == 1;
+    return false;
+}
+
 Tunables& Tunables::get()
 {
     static Tunables t = [] {
         Tunables x;
-        if (const char* e = getenv("GCDT_TAIL_THRESHOLD")) x.tail_threshold = atoi(e);
-        if (const char* e = getenv("GCDT_PIVOT")) x.pivot = atoi(e);
-        if (const char* e = getenv("GCDT_SINGLE_TESTER")) x.single_tester = atoi(e);
-        if (const char* e = getenv("GCDT_GRID_CAP")) g_grid_cap = std::max(256, std::min(65536, atoi(e)));
+        auto env = [&](const char* var, const char* name, int* dst) {
+            const char* e = getenv(var);
+            if (!e) return;
+            char* end = nullptr;
+            long v = strtol(e, &end, 10);
+            if (end == e || *end != 0 || v < INT_MIN || v > INT_MAX || !tunable_cleared(name, (int)v)) {
+                x.env_error += std::string(var) + "=" + e + " is not a cleared value; ";
+                return;
+            }
+            *dst = (int)v;
+        };
+        env("GCDT_TAIL_THRESHOLD", "tail_threshold", &x.tail_threshold);
+        env("GCDT_PIVOT", "pivot", &x.pivot);
+        env("GCDT_SINGLE_TESTER", "single_tester", &x.single_tester);
+        env("GCDT_GRID_CAP", "grid_cap", &g_grid_cap);
         return x;
     }();
     return t;
@@ -70,6 +92,7 @@ HostSlot take_host_slot()
 
 Ctx::Ctx(hipStream_t s) : st(s)
 {
+    if (!Tunables::get().env_error.empty()) throw Error(GCDT_ERR_INVALID, Tunables::get().env_error);
     tail_threshold = Tunables::get().tail_threshold;
     pivot = Tunables::get().pivot;
     single_tester = Tunables::get().single_tester;

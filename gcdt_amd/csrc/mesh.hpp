@@ -104,15 +104,20 @@ enum : int {
 };
 
 // Process-wide defaults of the round tunables (environment variables
-// GCDT_TAIL_THRESHOLD, GCDT_PIVOT, GCDT_SINGLE_TESTER override them at start).
+// GCDT_TAIL_THRESHOLD, GCDT_PIVOT, GCDT_SINGLE_TESTER and GCDT_GRID_CAP
+// override them at start; a value outside tunable_cleared() makes every
+// solve fail with GCDT_ERR_INVALID instead of running with it).
 struct Tunables {
     int tail_threshold = 4096;
     int pivot = 1;
     int single_tester = 1;
     int flip_items = 1;  // items per thread of flip test/apply: 1, 2 or 4
     int ins_items = 1;   // items per thread of insertion split/relocate: 1, 2 or 4
+    std::string env_error;
     static Tunables& get();
 };
+// whether a GPU parity test covers this value of the tunable
+bool tunable_cleared(const std::string& name, int value);
 
 struct Ctx {
     hipStream_t st = nullptr;

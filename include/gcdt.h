@@ -247,7 +247,9 @@ int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int 
  * "flip_items" / "ins_items" (1, 2 or 4 list items per thread of the flip
  * test/apply and insertion split/relocate launches), "pivot"
  * (0 random, 1 central-first, the default), "single_tester" (0/1).  Out-of-range values
- * fail with GCDT_ERR_INVALID.  gcdt_amd/tune.py picks tail_threshold,
+ * fail with GCDT_ERR_INVALID, and so do in-range values that no GPU parity
+ * test covers (the same rule applies to the GCDT_* environment overrides: a
+ * refused one makes every solve fail).  gcdt_amd/tune.py picks tail_threshold,
  * grid_cap and the items per thread by timing and caches the choice per
  * device. */
 int gcdt_set_tunable(const char* name, int value);
