@@ -1,0 +1,218 @@
+"""ctypes binding of the CPU SIMT emulator build (tests/emu/_build/libgcdt_emu.so)
+and a case runner for it (test infrastructure).
+
+The emulator build exports the library's C ABI; these helpers mirror
+gcdt_amd.api's solve / solve_batch over it.  Cases run in a child process
+(``python -m tests.emu.emu <case json>``), because a fault inside the
+emulator ends the process with status 86 after printing the kernel, the
+work-item and the allocation involved.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import importlib  # noqa: E402
+
+L = importlib.import_module("gcdt_amd.lib")  # signature table only; libgcdt.so is not loaded
+
+LIB = os.path.join(HERE, "_build", "libgcdt_emu.so")
+_lib = None
+
+
+def build():
+    """Builds the emulator library unless it is newer than every source
+    (serialised by a lock file, so parallel test workers build it once)."""
+    import fcntl
+    from tests.emu import build_emu
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    with open(os.path.join(HERE, "_build", ".lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        srcs = [os.path.join(build_emu.CSRC, f) for f in build_emu.SOURCES + build_emu.HEADERS]
+        srcs += [os.path.join(HERE, f) for f in ("emu_rt.cpp", "build_emu.py", os.path.join("hip", "hip_runtime.h"))]
+        if os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(f) for f in srcs):
+            return LIB
+        return build_emu.build()
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        d = ctypes.CDLL(LIB)
+        for name, (args, res) in L.SIGNATURES.items():
+            f = getattr(d, name, None)
+            if f is None:
+                continue
+            f.argtypes = args
+            f.restype = res
+        d.emu_set_tunable.argtypes = [ctypes.c_char_p, ctypes.c_int]
+        d.emu_set_tunable.restype = ctypes.c_int
+        d.emu_launches.restype = ctypes.c_ulonglong
+        _lib = d
+    return _lib
+
+
+def check(rc, where):
+    if rc != 0:
+        msg = load().gcdt_last_error()
+        raise L.GcdtError(rc, where, msg.decode() if msg else "")
+
+
+def set_knob(name: str, value: int):
+    if load().emu_set_tunable(name.encode(), int(value)) != 0:
+        raise ValueError(name)
+
+
+class Buf:
+    def __init__(self, nbytes):
+        self.nbytes = max(int(nbytes), 1)
+        p = ctypes.c_void_p()
+        check(load().gcdt_alloc(ctypes.byref(p), self.nbytes), "gcdt_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def of(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            check(load().gcdt_memcpy_h2d(b.ptr, a.ctypes.data, a.nbytes), "h2d")
+        return b
+
+    def get(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes:
+            check(load().gcdt_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().gcdt_release(self.ptr, self.nbytes)
+            self.ptr = None
+
+
+def solve(g, cons):
+    g = np.ascontiguousarray(g, np.int32).reshape(-1, 2)
+    c = np.ascontiguousarray(cons, np.int32).reshape(-1, 2)
+    n = len(g)
+    dp, dt, da = Buf.of(g), Buf(12 * (2 * n + 2)), Buf(12 * (2 * n + 2))
+    dc = Buf.of(c) if len(c) else None
+    nt = ctypes.c_int(0)
+    st = L.Stats()
+    check(load().gcdt_solve(dp.ptr, n, dc.ptr if dc else None, len(c), dt.ptr, da.ptr, ctypes.byref(nt),
+                            ctypes.byref(st), None), "gcdt_solve")
+    tris = dt.get(np.int32, (nt.value, 3))
+    adj = da.get(np.int32, (nt.value, 3))
+    for b in (dp, dt, da, dc):
+        if b:
+            b.free()
+    return tris, adj, st.as_dict()
+
+
+def solve_batch(instances):
+    from gcdt_amd.api import pack_instances
+    flat_p, poff, flat_c, coff, toff = pack_instances(instances)
+    dp = Buf.of(flat_p)
+    dc = Buf.of(flat_c) if len(flat_c) else None
+    dt = Buf(12 * int(toff[-1]))
+    st = L.BatchStats()
+    P64 = ctypes.POINTER(ctypes.c_int64)
+    check(load().gcdt_solve_batch(dp.ptr, poff.ctypes.data_as(P64), len(instances), dc.ptr if dc else None,
+                                  coff.ctypes.data_as(P64) if dc else None, dt.ptr, ctypes.byref(st), None),
+          "gcdt_solve_batch")
+    allt = dt.get(np.int32, (int(toff[-1]), 3))
+    for b in (dp, dt, dc):
+        if b:
+            b.free()
+    return [allt[toff[i]:toff[i + 1]] for i in range(len(instances))], st.as_dict()
+
+
+# ------------------------------------------------------------------ cases
+
+def _instances(spec):
+    from gcdt_amd import gen
+    if "count" in spec:  # the batched workload's star instances
+        inst = gen.batched_instances(count=spec["count"], nmin=spec.get("nmin", 64), nmax=spec.get("nmax", 65536),
+                                     seed=spec.get("seed", 6))
+        return [(gen.to_grid(p), c) for p, c in inst]
+    s = gen.Stream(spec.get("seed", 7))
+    out = []
+    for n in spec["sizes"]:
+        p, c = gen.polygon_instance(s, int(n))
+        out.append((gen.to_grid(p), c))
+    return out
+
+
+def run_case(case: dict) -> dict:
+    """One emulated solve with the given knobs, validated by the oracle."""
+    from gcdt_amd import gen
+    from oracle import oracle as O
+    for k, v in case.get("knobs", {}).items():
+        set_knob(k, v)
+    res = {"valid": True, "unique_mismatch": 0}
+    kind = case["kind"]
+    if kind == "batch":
+        inst = _instances(case)
+        tris, st = solve_batch(inst)
+        bad = []
+        for i, ((g, c), t) in enumerate(zip(inst, tris)):
+            v = O.validate(g, t, c)
+            if not v["valid"]:
+                bad.append(i)
+        res["valid"] = not bad
+        res["bad"] = bad
+        res["stats"] = {k: st[k] for k in ("insert_rounds", "flip_rounds", "constraint_rounds", "num_triangles")}
+    else:
+        if kind == "uniform":
+            p, c = gen.uniform_cdt(n=case["n"], c=case.get("c", 0), seed_pts=case.get("seed", 1),
+                                   seed_cons=case.get("seed", 1) + 1)
+        elif kind == "lattice":
+            p, c = gen.lattice_cdt(k=case["k"], c=case.get("c", 0), max_len=case.get("max_len", 8))
+        elif kind == "clustered":
+            p, c = gen.clustered_cdt(n=case["n"], circles=case.get("circles", 0))
+        else:
+            raise ValueError(kind)
+        g = gen.to_grid(p)
+        tris, adj, st = solve(g, c)
+        v = O.validate(g, tris, c)
+        res["valid"] = bool(v["valid"])
+        if v["valid"] and v.get("unique"):
+            ref, _ = O.cdt(g, c)
+            res["unique_mismatch"] = int(not np.array_equal(O.canonical(tris), O.canonical(ref)))
+        res["stats"] = {k: st[k] for k in ("insert_rounds", "flip_rounds", "constraint_rounds", "num_triangles")}
+    res["launches"] = int(load().emu_launches())
+    return res
+
+
+def run_in_child(case: dict, seed: int = 1, align_low: int = 0, conc: int = 1, yield_atomics: int = 0,
+                 timeout: float = 600.0):
+    """Runs `case` in a child process; returns (exit status, parsed result or
+    None, stderr tail).  seed: order of workgroups and work-items and the
+    fill of new allocations; align_low: allocations against their lower guard
+    page; conc: workgroups interleaved at once; yield_atomics: switch
+    work-items at every atomic."""
+    env = dict(os.environ, EMU_SEED=str(seed), EMU_ALIGN_LOW=str(align_low), EMU_CONC=str(conc),
+               EMU_YIELD_ATOMICS=str(yield_atomics))
+    r = subprocess.run([sys.executable, "-m", "tests.emu.emu", json.dumps(case)], cwd=ROOT, env=env,
+                       capture_output=True, text=True, timeout=timeout)
+    out = None
+    for line in r.stdout.splitlines():
+        if line.startswith("EMU_RESULT "):
+            out = json.loads(line[len("EMU_RESULT "):])
+    return r.returncode, out, r.stderr[-3000:]
+
+
+if __name__ == "__main__":
+    case = json.loads(sys.argv[1])
+    print("EMU_RESULT " + json.dumps(run_case(case)), flush=True)

@@ -1,0 +1,95 @@
+"""The library's kernels and round drivers executed on the CPU by the SIMT
+emulator (tests/emu): every device allocation sits between guard pages, new
+allocations hold random bytes, workgroups run interleaved and work-items
+switch at every atomic, so an out-of-bounds list write, a read of an unwritten
+entry used as an index, or a pool release with the wrong size ends the child
+process with the kernel and work-item named.
+
+Covered: every value of every round tunable that the autotuner sweeps (and
+grid caps far below the GPU's, so grid-stride loops run many iterations),
+on mixed-size batches, single solves with constraints, and the degenerate
+lattice.  Each result must also pass the oracle's exact validator."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+from tests.emu import emu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SWEEP = {
+    "tail_threshold": [0, 16, 512, 1024, 2048, 8192, 16384, 1 << 20],
+    "grid_cap": [1, 3, 2048, 16384],
+    "flip_items": [2, 4],
+    "ins_items": [2, 4],
+    "pivot": [0],
+    "single_tester": [0],
+}
+BATCH = [64, 300, 2000, 900, 150, 5000, 70, 12000]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    emu.build()
+
+
+def _run(case, **kw):
+    rc, out, err = emu.run_in_child(case, **kw)
+    assert rc == 0, f"emulated run failed with status {rc}:\n{err}"
+    assert out is not None and out["valid"], out
+    assert out["unique_mismatch"] == 0, out
+    return out
+
+
+@pytest.mark.parametrize("knob,value", [(k, v) for k, vs in SWEEP.items() for v in vs])
+def test_batch_at_every_swept_value(knob, value):
+    _run({"kind": "batch", "sizes": BATCH, "seed": 7, "knobs": {knob: value}},
+         seed=hash((knob, value)) % 1000 + 1, align_low=value & 1, conc=8, yield_atomics=1)
+
+
+def test_batch_all_knobs_off_default():
+    knobs = {"tail_threshold": 64, "grid_cap": 3, "flip_items": 2, "ins_items": 4, "pivot": 0, "single_tester": 0}
+    for seed in (11, 12):
+        _run({"kind": "batch", "sizes": BATCH, "seed": seed, "knobs": knobs}, seed=seed, align_low=seed & 1, conc=16,
+             yield_atomics=1)
+
+
+def test_star_instances():
+    _run({"kind": "batch", "count": 6, "nmax": 20000, "seed": 6}, seed=5, conc=8, yield_atomics=1)
+
+
+@pytest.mark.parametrize("tail", [0, 4096, 1 << 20])
+def test_single_solve_with_constraints(tail):
+    _run({"kind": "uniform", "n": 20000, "c": 200, "knobs": {"tail_threshold": tail}}, seed=tail % 97 + 1,
+         align_low=1, conc=8, yield_atomics=1)
+
+
+def test_lattice_degenerate():
+    _run({"kind": "lattice", "k": 48, "c": 40, "max_len": 20, "knobs": {"grid_cap": 2}}, seed=3, conc=4,
+         yield_atomics=1)
+
+
+def test_clustered_fans():
+    _run({"kind": "clustered", "n": 30000, "circles": 10}, seed=9, conc=8, yield_atomics=1)
+
+
+@pytest.mark.parametrize("where", ["past_end", "before_start"])
+def test_detector_catches_an_out_of_bounds_write(where):
+    """The guard pages do fault: a write one element past the end (or before
+    the start) of an allocation ends the child with status 86 and names the
+    allocation and the offset."""
+    code = ("import ctypes; from tests.emu import emu; L = emu.load(); "
+            f"L.emu_selftest_oob.argtypes = [ctypes.c_int]; L.emu_selftest_oob({1 if where == 'past_end' else -1})")
+    env = dict(os.environ, EMU_ALIGN_LOW="0" if where == "past_end" else "1")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 86, (r.returncode, r.stderr)
+    assert "emu_selftest" in r.stderr and "offset" in r.stderr, r.stderr
+
+
+def test_detector_catches_a_wrong_size_release():
+    code = "from tests.emu import emu; L = emu.load(); L.emu_selftest_release()"
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 91, (r.returncode, r.stderr)
