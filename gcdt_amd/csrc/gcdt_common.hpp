@@ -10,6 +10,16 @@
 
 #include "../../include/gcdt.h"
 
+// A point between the separate memory instructions of one multi-word record
+// load or store.  It generates nothing: on the GPU another work-item's
+// accesses can land between those instructions anyway (a record read
+// concurrently with its rewrite may see some 16-byte parts old and others
+// new).  The CPU SIMT emulator of the test suite defines it as a switch
+// between work-items, so that such interleavings happen on the host too.
+#ifndef GCDT_MEM_POINT
+#define GCDT_MEM_POINT()
+#endif
+
 namespace gcdt {
 
 // hipError_t -> exception carrying a GCDT status; caught at the C-ABI edge.

@@ -441,7 +441,13 @@ __host__ __device__ inline int rec_with_mark(int meta, int epoch)
 __host__ __device__ inline FRec rec_load(const FRec* p)
 {
     const int4* q = reinterpret_cast<const int4*>(p);
-    int4 a = q[0], b = q[1], c = q[2], d = q[3];
+    int4 a = q[0];
+    GCDT_MEM_POINT();
+    int4 b = q[1];
+    GCDT_MEM_POINT();
+    int4 c = q[2];
+    GCDT_MEM_POINT();
+    int4 d = q[3];
     FRec r;
     r.v = make_int3(a.x, a.y, a.z);
     r.a = make_int3(a.w, b.x, b.y);
@@ -459,8 +465,11 @@ __host__ __device__ inline void rec_store(FRec* p, const int3& v, const int3& a,
 {
     int4* q = reinterpret_cast<int4*>(p);
     q[0] = make_int4(v.x, v.y, v.z, a.x);
+    GCDT_MEM_POINT();
     q[1] = make_int4(a.y, a.z, p0.x, p0.y);
+    GCDT_MEM_POINT();
     q[2] = make_int4(p1.x, p1.y, p2.x, p2.y);
+    GCDT_MEM_POINT();
     q[3] = make_int4(meta, stamp, (int)(unsigned)claim, (int)(unsigned)(claim >> 32));
 }
 
@@ -623,27 +632,32 @@ __host__ __device__ inline bool flip_fixtest_p(const FlipP& f, int t, unsigned e
 // queued).  The two outer slots of each keep the old neighbours' ids; the
 // stamp marks them for settling by the next round's test (flip_fixtest_p) or
 // by flip_fix_p.
+//
+// When both triangles of an edge test it (the persistent tail, or
+// single_tester 0), a proposal can lose its own triangle to another one that
+// flips it in this same pass.  Its record may then be read while that flip
+// stores it, 16 bytes at a time: some parts new, some old.  Only the claim
+// word is the same in every such view (the winning key), so nothing else of
+// the record -- above all no neighbour id, which may be -1 in the new parts
+// -- is used before the claim says t is still its own.
 template <bool COH = true, class CL = RecClaims>
 __host__ __device__ inline int flip_apply_p(const FlipP& f, int t, unsigned epoch, int* nout, const CL& cl = CL())
 {
-    // whole records first: the proposal and both claim words lie in the two
-    // lines the flip needs anyway (inside a persistent workgroup the claims
-    // are re-read with atomic loads)
     const FRec r = rec_load(&f.r[t]);
     const int k = rec_cand(r.meta);
     if (k == 3) return 0;
-    const int n = get3(r.a, k);
-    const FRec rn = rec_load(&f.r[n]);
     const unsigned long long key = ((unsigned long long)epoch << 32) | (unsigned)(3 * t + k);
     const unsigned long long ct = cl.template get<COH>(f, t, r.claim);
-    const unsigned long long cn = cl.template get<COH>(f, n, rn.claim);
-    if (ct != key || cn != key) {
-        if (ct != key) {
-            int t2 = (int)((ct & 0xffffffffu) / 3);
-            if ((ct >> 32) == epoch && cl.template load<COH>(f, t2) == ct) return 3;
-        }
+    if (ct != key) {
+        int t2 = (int)((ct & 0xffffffffu) / 3);
+        if ((ct >> 32) == epoch && cl.template load<COH>(f, t2) == ct) return 3;
         return 1;
     }
+    // t holds its own claim, so no other flip writes its record in this pass
+    const int n = get3(r.a, k);
+    const FRec rn = rec_load(&f.r[n]);
+    const unsigned long long cn = cl.template get<COH>(f, n, rn.claim);
+    if (cn != key) return 1;
     const int kn = nx3(k), kp = pv3(k);
     const int p = get3(r.v, k), u = get3(r.v, kn), v = get3(r.v, kp);
     const int j = slot_other(rn.v, u, v);

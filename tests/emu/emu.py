@@ -196,14 +196,15 @@ def run_case(case: dict) -> dict:
 
 
 def run_in_child(case: dict, seed: int = 1, align_low: int = 0, conc: int = 1, yield_atomics: int = 0,
-                 timeout: float = 600.0):
+                 timeout: float = 600.0, yield_mem: int = 0):
     """Runs `case` in a child process; returns (exit status, parsed result or
     None, stderr tail).  seed: order of workgroups and work-items and the
     fill of new allocations; align_low: allocations against their lower guard
     page; conc: workgroups interleaved at once; yield_atomics: switch
-    work-items at every atomic."""
+    work-items at every atomic; yield_mem: also between the 16-byte parts
+    of a record load or store."""
     env = dict(os.environ, EMU_SEED=str(seed), EMU_ALIGN_LOW=str(align_low), EMU_CONC=str(conc),
-               EMU_YIELD_ATOMICS=str(yield_atomics))
+               EMU_YIELD_ATOMICS=str(yield_atomics), EMU_YIELD_MEM=str(yield_mem))
     r = subprocess.run([sys.executable, "-m", "tests.emu.emu", json.dumps(case)], cwd=ROOT, env=env,
                        capture_output=True, text=True, timeout=timeout)
     out = None

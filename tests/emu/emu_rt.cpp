@@ -151,6 +151,7 @@ struct Fiber {
     int op = 0;
     unsigned long long val = 0, res = 0;
     int arg = 0;
+    int stall = 0;  // scheduler passes to skip this work-item (EMU_YIELD_MEM=2)
 };
 
 struct Slot {
@@ -174,6 +175,8 @@ static const U3 kZero{0, 0, 0}, kOne{1, 1, 1};
 static std::recursive_mutex g_launch_mu;  // launches from several host threads take turns
 static int g_conc = 1;                    // workgroups interleaved at once
 static int g_yield_atomics = 0;           // switch work-items at every atomic
+static thread_local unsigned long long t_order_state = 1;  // scheduling order
+static int g_yield_mem = 0;               // ... and between the 16-byte parts of a record load or store
 
 const U3& cur_tid() { return t_exec && t_exec->cur ? t_exec->cur->tid : kZero; }
 const U3& cur_bid() { return t_exec && t_exec->cur ? t_exec->slots[t_exec->cur->slot].bid : kZero; }
@@ -192,6 +195,16 @@ void atomic_point()
     Exec* x = t_exec;
     if (!g_yield_atomics || !x || !x->cur) return;
     yield_to_scheduler();  // stays READY: another work-item may run first
+}
+
+void mem_point()
+{
+    Exec* x = t_exec;
+    if (!g_yield_mem || !x || !x->cur) return;
+    // mode 2: now and then the work-item also stalls for up to 64 passes, so
+    // that others run whole bodies between two parts of its record access
+    if (g_yield_mem >= 2 && splitmix(t_order_state) % 8 == 0) x->cur->stall = 1 + (int)(splitmix(t_order_state) % 64);
+    yield_to_scheduler();
 }
 
 [[noreturn]] static void fiber_main()
@@ -227,6 +240,7 @@ static void fiber_prepare(Fiber& f)
     for (int i = 0; i < 6; i++) *--sp = nullptr;
     f.sp = sp;
     f.st = READY;
+    f.stall = 0;
 }
 
 void block_barrier()
@@ -311,7 +325,6 @@ static bool release_wave(Exec* x, int s, int w)
     return true;
 }
 
-static thread_local unsigned long long t_order_state = 1;
 
 static void start_block(Exec* x, int s, unsigned b)
 {
@@ -367,6 +380,10 @@ void run_grid(const char* name, const LaunchCfg& cfg, Body& body)
         for (int k = 0; k < N; k++) {
             Fiber& f = x->fibers[order[k]];
             if (!x->slots[f.slot].active || f.st != READY) continue;
+            if (f.stall > 0) {
+                f.stall--;
+                continue;
+            }
             x->cur = &f;
             emu_ctx_switch(&x->sched_sp, f.sp);
             x->cur = nullptr;
@@ -432,6 +449,7 @@ struct Init {
         if (const char* e = getenv("EMU_ALIGN_LOW")) g_align_low = atoi(e);
         if (const char* e = getenv("EMU_CONC")) g_conc = std::max(1, atoi(e));
         if (const char* e = getenv("EMU_YIELD_ATOMICS")) g_yield_atomics = atoi(e);
+        if (const char* e = getenv("EMU_YIELD_MEM")) g_yield_mem = atoi(e);
         t_order_state = g_seed * 0x9E3779B97F4A7C15ull + 1;
         g_fill_state ^= g_seed;
         struct sigaction sa;

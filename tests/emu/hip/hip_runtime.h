@@ -56,6 +56,7 @@ const U3& cur_gdim();
 void block_barrier();
 int cur_slot();
 void atomic_point();  // a switch point before every atomic (when enabled)
+void mem_point();     // a switch point between the parts of a record access (when enabled)
 constexpr int kMaxConc = 16;
 template <class T, int ID>
 inline T& lds()
@@ -105,6 +106,7 @@ void launch(const char* name, const LaunchCfg& cfg, void (*k)(P...), A&&... a)
 #define blockIdx (::emu::cur_bid())
 #define blockDim (::emu::cur_bdim())
 #define gridDim (::emu::cur_gdim())
+#define GCDT_MEM_POINT() ::emu::mem_point()
 
 inline void __syncthreads() { ::emu::block_barrier(); }
 inline int __lane_id() { return (int)(threadIdx.x & 63u); }

@@ -11,6 +11,8 @@
 #include <cstring>
 #include <vector>
 
+#include <sys/mman.h>
+
 namespace gcdt {
 __host__ inline unsigned long long ld_u64(const unsigned long long* p) { return *p; }
 __host__ inline unsigned long long at_max(unsigned long long* p, unsigned long long v) { auto o = *p; if (v > o) *p = v; return o; }
@@ -33,6 +35,11 @@ static unsigned g_epoch = 0;
 static unsigned g_seed = 0;  // item-order shuffles
 static int g_fused = 1;      // packed rounds settle flips inside the next test
 static int g_two = 0;        // 1: both triangles of an edge test it in every round (else only short lists)
+// 1: a proposer whose record another proposal's flip has rewritten earlier in
+// the same apply pass reads it torn -- vertices, neighbours and coordinates
+// new, meta/stamp/claim old -- as a GPU load of the record can when it runs
+// concurrently with that flip's four 16-byte stores
+static int g_torn = 0;
 
 static void shuffle(std::vector<int>& v, unsigned salt)
 {
@@ -135,16 +142,35 @@ static long long flip_rounds(const Mesh& m, int T, int* rounds)
 // which case flip_fix_p runs after every apply.  Tests and applies visit the
 // list in shuffled orders, so retries meet flipped neighbours both before and
 // after those neighbours' own settling.
+// Records in a mapping whose first page is inaccessible and holds no record,
+// so a read of rec[-1] (a neighbour id of -1 used as an index) faults.
+struct GuardedRecs {
+    char* base = nullptr;
+    size_t bytes = 0;
+    FRec* r = nullptr;
+    explicit GuardedRecs(int T)
+    {
+        const size_t page = 4096;
+        bytes = page + ((size_t)std::max(T, 1) * sizeof(FRec) + page - 1) / page * page;
+        base = (char*)mmap(nullptr, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+        if (base == MAP_FAILED) abort();
+        mprotect(base, page, PROT_NONE);
+        r = (FRec*)(base + page);
+    }
+    ~GuardedRecs() { munmap(base, bytes); }
+    FRec& operator[](long long i) { return r[i]; }
+};
+
 static long long flip_rounds_p(const Mesh& m, int T, int* rounds)
 {
-    std::vector<FRec> rec(T);
+    GuardedRecs rec(T);
     for (int t = 0; t < T; t++) {
         int3 v = m.tri[t];
         rec_store(&rec[t], v, m.adj[t], m.pts[v.x], m.pts[v.y], m.pts[v.z],
                   (m.cflag ? m.cflag[t] & 7 : 0) | FREC_NOCAND | FREC_NOPARTNER, 0, 0ull);
     }
     std::vector<int> qround(T, 0);
-    FlipP f{rec.data(), &g_err};
+    FlipP f{rec.r, &g_err};
     std::vector<int> act(T);
     for (int i = 0; i < T; i++) act[i] = i;
     long long flips = 0;
@@ -160,9 +186,21 @@ static long long flip_rounds_p(const Mesh& m, int T, int* rounds)
         }
         std::vector<int> nx, fl;
         shuffle(ord, ~ep);
+        std::vector<FRec> before;
+        if (g_torn) before.assign(rec.r, rec.r + T);
         for (int t : ord) {
             int n = -1;
-            int res = flip_apply_p(f, t, ep, &n);
+            int res;
+            if (g_torn && memcmp(&rec[t], &before[t], sizeof(FRec)) != 0) {
+                // rewritten by another proposal's flip in this pass
+                const FRec now = rec[t];
+                memcpy((char*)&rec[t] + 48, (const char*)&before[t] + 48, 16);
+                res = flip_apply_p(f, t, ep, &n);
+                if (res == 2) return -1;  // t flipped from a torn read
+                rec[t] = now;
+            } else {
+                res = flip_apply_p(f, t, ep, &n);
+            }
             if (res == 1) rec[t].meta = rec_with_mark(rec[t].meta, (int)ep + 1);
             if (res == 1 || res == 2) nx.push_back(t);
             if (res == 2) {
@@ -230,6 +268,8 @@ static void dag_init(DAG& d, int cap, int N)
 // Full solve on the host: sample mesh, located insertion, packed flip rounds,
 // constraint cavities and the constrained flip pass.  Returns 0 or an error
 // code (the hundreds say which stage failed); tris_out holds 2N+2 triangles.
+extern "C" void host_set_torn(int on) { g_torn = on; }
+
 extern "C" int host_solve(const int* xy, int N, const int* consp, int C, int* tris_out, long long* stats,
                           unsigned seed, int fused, int two_testers)
 {

@@ -37,6 +37,9 @@ def _built():
 
 
 def _run(case, **kw):
+    """yield_mem=2 also switches work-items between the 16-byte parts of
+    record loads and stores, stalling some for many passes, so records are
+    read while flips rewrite them."""
     rc, out, err = emu.run_in_child(case, **kw)
     assert rc == 0, f"emulated run failed with status {rc}:\n{err}"
     assert out is not None and out["valid"], out
@@ -47,7 +50,7 @@ def _run(case, **kw):
 @pytest.mark.parametrize("knob,value", [(k, v) for k, vs in SWEEP.items() for v in vs])
 def test_batch_at_every_swept_value(knob, value):
     _run({"kind": "batch", "sizes": BATCH, "seed": 7, "knobs": {knob: value}},
-         seed=hash((knob, value)) % 1000 + 1, align_low=value & 1, conc=8, yield_atomics=1)
+         seed=hash((knob, value)) % 1000 + 1, align_low=value & 1, conc=8, yield_atomics=1, yield_mem=2)
 
 
 def test_batch_all_knobs_off_default():

@@ -67,3 +67,45 @@ def test_two_testers_per_edge(seed):
     p, c = gen.uniform_cdt(n=12_000, c=120, seed_pts=500 + seed, seed_cons=600 + seed)
     _check(gen.to_grid(p), c, seed, two_testers=True)
     _check(gen.to_grid(p), c, seed, fused=False, two_testers=True)
+
+
+_TORN_CHILD = r"""
+import sys
+import numpy as np
+from gcdt_amd import gen
+from oracle import oracle as O
+from tests import host_bodies as HB
+HB.set_torn(True)
+bad = []
+for seed in range(int(sys.argv[1]), int(sys.argv[2])):
+    rng = np.random.default_rng(seed)
+    p, _ = gen.dedup(rng.integers(0, 1 << 20, size=(int(rng.integers(3, 12)), 2)))
+    g = gen.to_grid(p)
+    for s in range(2):
+        rc, tris, st = HB.solve(g, np.zeros((0, 2), np.int32), seed=s, two_testers=True)
+        if rc != 0 or not O.validate(g, tris)["valid"]:
+            bad.append((seed, s, rc))
+print("BAD", bad)
+"""
+
+
+def test_torn_record_reads_by_losing_proposals():
+    """Both triangles of an edge test it, and a proposal whose triangle another
+    proposal flips earlier in the same apply pass reads its own record torn:
+    vertices, neighbours and coordinates already rewritten, meta and claim
+    not yet -- as a GPU load of the record can while that flip's 16-byte
+    stores land.  The new neighbour in the proposal's slot can be -1 (a
+    square edge), and the records sit after an inaccessible page, so an apply
+    that follows that neighbour before checking its own claim reads record
+    -1 and the child faults.  Tiny inputs (3-11 points) make the square's
+    edges common; seeds 57 and 69 hit the -1 case.  The unoptimised build
+    makes every load the source names, as the GPU code does."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, HOST_BODIES_O0="1")
+    r = subprocess.run([sys.executable, "-c", _TORN_CHILD, "0", "400"], cwd=root, env=env, capture_output=True,
+                       text=True, timeout=600)
+    assert r.returncode == 0, (r.returncode, r.stderr[-2000:])
+    assert "BAD []" in r.stdout, r.stdout
