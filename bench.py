@@ -261,7 +261,7 @@ def main(argv=None):
         tc = time.perf_counter()
         O.cdt(g, cons)
         dtc = time.perf_counter() - tc
-        cpu = {"value": round(n / dtc / 1e6, 4), "unit": "Mpts/s", "cores": 1, "kind": "port",
+        cpu = {"value": round(n / dtc / 1e6, 4), "unit": "Mpts/s", "cores": 1, "kind": "port", "host": _host_cpus(1),
                "sample": f"the full workload ({n} points, {c} segments), serial exact-predicate CDT, "
                          f"one host core, {dtc:.2f} s"}
 
@@ -337,17 +337,8 @@ def bench_batched(args):
     check(L.gcdt_synchronize(), "sync")
     ms_batch = (time.perf_counter() - tb) * 1000.0 / reps
     print(f"# batched solve: {ms_batch:.1f} ms ({st})", file=sys.stderr, flush=True)
-    # exactness of a sample of instances
-    O = _checker(args)
-    checked = 0
-    bad = 0
-    if O is not None:
-        allt = dt.to_array(np.int32, (int(toff[-1]), 3))
-        for i in range(0, len(inst), max(1, len(inst) // 32)):
-            g, c = inst[i]
-            v = O.validate(g, allt[toff[i]:toff[i + 1]], c)
-            checked += 1
-            bad += 0 if v["valid"] else 1
+    allt = dt.to_array(np.int32, (int(toff[-1]), 3))
+    results = [allt[toff[i]:toff[i + 1]] for i in range(len(inst))]
     for b in (dp, dc, dt):
         b.free()
     # one plain solve per instance
@@ -380,10 +371,12 @@ def bench_batched(args):
     batched_cdts = len(inst) / (ms_batch / 1000.0)
     single_cdts = m / (ms_single / 1000.0)
     cpu = None
+    refs = None
     if not args.no_cpu:
-        rate, thr, secs = _cpu_pool_cdts(inst)
-        cpu = {"value": round(rate, 1), "unit": "CDTs/s", "cores": thr, "kind": "port",
+        rate, thr, secs, refs = _cpu_pool_cdts(inst)
+        cpu = {"value": round(rate, 1), "unit": "CDTs/s", "cores": thr, "kind": "port", "host": _host_cpus(thr),
                "sample": f"all {len(inst)} instances, one per host thread, serial exact-predicate CDT, {secs:.2f} s"}
+    exact = _check_all(_checker(args), inst, results, refs)
     # the per-instance rate is measured on the first m instances; scale by point
     # counts too so an unrepresentative subset is visible
     return {"instances": len(inst), "points": int(len(flat_p)), "segments": int(len(flat_c)),
@@ -395,14 +388,34 @@ def bench_batched(args):
             "speedup_cdts": round(batched_cdts / single_cdts, 2),
             "rounds": {"insert": st["insert_rounds"], "flip": st["flip_rounds"],
                        "constraints": st["constraint_rounds"]},
-            "validated_instances": checked, "invalid_instances": bad, "cpu_baseline": cpu,
+            "exactness": exact, "cpu_baseline": cpu,
             "tunables": {"batched": tuned_b["choice"] if tuned_b else "defaults",
                          "per_instance": tuned_s["choice"] if tuned_s else "defaults"}}
 
 
-def _cpu_pool_cdts(instances, max_threads=16):
+# Host threads of the CPU legs: the host cores this run may use, at most 16
+# (the share of a one-GPU machine's host; nproc there may report more).
+CPU_THREADS = 16
+
+
+def _host_cpus(used):
+    """nproc, the CPU model and the threads the CPU leg used."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model, "threads_used": used}
+
+
+def _cpu_pool_cdts(instances, max_threads=CPU_THREADS):
     """The serial oracle over every instance, one instance per host thread
-    (the ctypes call releases the GIL).  Returns (CDTs/s, threads, seconds)."""
+    (the ctypes call releases the GIL).  Returns (CDTs/s, threads, seconds,
+    the oracle's triangle lists)."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import oracle as O
@@ -410,9 +423,40 @@ def _cpu_pool_cdts(instances, max_threads=16):
     threads = max(1, min(max_threads, os.cpu_count() or 1))
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        list(ex.map(lambda gc: O.cdt(gc[0], gc[1]), instances))
+        refs = [r[0] for r in ex.map(lambda gc: O.cdt(gc[0], gc[1]), instances)]
     dt = time.perf_counter() - t0
-    return len(instances) / dt, threads, dt
+    return len(instances) / dt, threads, dt, refs
+
+
+def _check_all(O, instances, results, refs=None, max_threads=CPU_THREADS):
+    """Every GPU result through the oracle's exact validator, and compared
+    with the oracle's canonical triangle set where the CDT is unique (refs:
+    the oracle's triangles per instance, when the CPU leg computed them).
+    Runs after the timed regions."""
+    if O is None:
+        return None
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    def one(i):
+        (g, c), tris = instances[i], results[i]
+        if tris.shape != (2 * len(g) + 2, 3):
+            return "invalid", False
+        v = O.validate(g, tris, c)
+        if not v["valid"]:
+            return "invalid", False
+        if not v["unique"]:
+            return "valid", False
+        ref = refs[i] if refs is not None else O.cdt(g, c)[0]
+        return ("equal" if np.array_equal(O.canonical(tris), O.canonical(ref)) else "differs"), True
+
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(one, range(len(instances))))
+    return {"checked": len(res), "invalid": sum(r == "invalid" for r, _ in res),
+            "unique": sum(u for _, u in res), "equal_to_oracle": sum(r == "equal" for r, _ in res),
+            "differ_from_oracle": sum(r == "differs" for r, _ in res)}
 
 
 def bench_degenerate(args):
@@ -481,18 +525,10 @@ def bench_serving(args):
     srv = S.Server(workers=args.serve_workers, small_n=args.serve_small_n)
     st = srv.run(reqs)
     summ = st.summary(args.serve_qps)
-    O = _checker(args)
-    bad = 0
-    checked = 0
-    for r in (reqs[:: max(1, len(reqs) // 16)] if O is not None else []):
-        v = O.validate(r.pts, r.fetch(), r.cons)
-        checked += 1
-        bad += 0 if v["valid"] else 1
     for r in reqs:
         r.fetch()
     summ.update({"points": int(npts), "workers": args.serve_workers, "small_n": srv.small_n,
                  "mpts_per_s": round(npts / max(st.makespan_s, 1e-9) / 1e6, 3),
-                 "validated_requests": checked, "invalid_requests": bad,
                  "latency": "arrival to triangles resident on the device"})
     # saturation: every request waiting at t=0; continuous batching on the
     # worker streams against one plain solve at a time on one stream
@@ -514,11 +550,16 @@ def bench_serving(args):
                          "p99_ms": sat.summary(0)["p99_ms"],
                          "one_at_a_time_cdts_per_s": round(one_rate, 2),
                          "speedup_cdts": round(sat_rate / max(one_rate, 1e-9), 2)}
+    refs = None
     if not args.no_cpu:
-        rate, thr, secs = _cpu_pool_cdts([(r.pts, r.cons) for r in reqs])
+        rate, thr, secs, refs = _cpu_pool_cdts([(r.pts, r.cons) for r in reqs])
         summ["cpu_baseline"] = {"value": round(rate, 2), "unit": "CDTs/s", "cores": thr, "kind": "port",
+                                "host": _host_cpus(thr),
                                 "sample": f"all {len(reqs)} requests, one per host thread, serial exact-predicate "
                                           f"CDT, {secs:.2f} s"}
+    # every request of the timed run (merged or alone) against the oracle
+    summ["exactness"] = _check_all(_checker(args), [(r.pts, r.cons) for r in reqs], [r.fetch() for r in reqs],
+                                   refs)
     print(f"# serving: {summ}", file=sys.stderr, flush=True)
     return summ
 
