@@ -94,6 +94,8 @@ def _prof(lib, name):
 
 
 def main(argv=None):
+    import faulthandler
+    faulthandler.enable()  # a crash prints every thread's Python stack
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -108,24 +110,16 @@ def main(argv=None):
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the solves, copy MIB MiB device-to-device 3 times (a plain copy kernel of "
                          "known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
-    ap.add_argument("--no-tune", action="store_true",
-                    help="keep the default tunables (always the case: the autotuner sweep is switched off)")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tunables (no autotuner sweep)")
     ap.add_argument("--tune", action="store_true",
-                    help="run the autotuner sweep (switched off: a batched solve made an illegal memory access "
-                         "during one sweep, trial unknown; exits with an error)")
+                    help="run the autotuner sweep even when a cached choice exists for this device")
     ap.add_argument("--no-degenerate", action="store_true", help="skip the 4096^2 lattice validity run")
-    ap.add_argument("--serve-count", type=int, default=128, help="requests in the serving run (0 = skip)")
+    ap.add_argument("--serve-count", type=int, default=256, help="requests in the serving run (0 = skip)")
     ap.add_argument("--serve-qps", type=float, default=50.0, help="Poisson arrival rate of the serving run")
     ap.add_argument("--serve-workers", type=int, default=3, help="worker streams of the serving run")
     ap.add_argument("--serve-small-n", type=int, default=65536,
                     help="requests up to this many points are merged into batched solves")
     args = ap.parse_args(argv)
-    if args.tune:
-        print("bench.py: the autotuner sweep is switched off (a batched solve made an illegal memory access "
-              "during one sweep of non-default tunables; the trial is not known). Run without --tune.",
-              file=sys.stderr)
-        sys.exit(2)
-    args.no_tune = True
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X only: this CDT path has no multi-GPU code (the sharding design "
               "is documented, not built). Run with --gpus 1.",
@@ -162,7 +156,7 @@ def main(argv=None):
         def run_large():
             solve_device(dp, n, dc, c, dt, da)
 
-        tuned = tune.load_or_tune("large", run_large)
+        tuned = tune.load_or_tune("large", run_large, force=args.tune)
         print(f"# tunables (large): {tuned}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         T, st = solve_device(dp, n, dc, c, dt, da)
@@ -326,9 +320,16 @@ def bench_batched(args):
     if not args.no_tune:
         from gcdt_amd import tune
 
-        tuned_b = tune.load_or_tune("batched", lambda: solve_batch_device(dp, poff, dc, coff, dt))
+        tuned_b = tune.load_or_tune("batched", lambda: solve_batch_device(dp, poff, dc, coff, dt), force=args.tune)
         print(f"# tunables (batched): {tuned_b}", file=sys.stderr, flush=True)
+    # the pool's peak over one solve, this workload's inputs and output included
+    check(L.gcdt_synchronize(), "sync")
+    check(L.gcdt_pool_reset_peak(), "pool")
     st = solve_batch_device(dp, poff, dc, coff, dt)  # warmup
+    import ctypes as _ct
+    _u, _c, _pk = _ct.c_size_t(0), _ct.c_size_t(0), _ct.c_size_t(0)
+    check(L.gcdt_pool_stats(_ct.byref(_u), _ct.byref(_c), _ct.byref(_pk)), "pool")
+    pool_peak = int(_pk.value)
     reps = 2
     check(L.gcdt_synchronize(), "sync")
     tb = time.perf_counter()
@@ -354,7 +355,7 @@ def bench_batched(args):
             for dpi, dci, dti, dai, nn, cc in bufs[:16]:
                 solve_device(dpi, nn, dci, cc, dti, dai)
 
-        tuned_s = tune.load_or_tune("per_instance", run_small)
+        tuned_s = tune.load_or_tune("per_instance", run_small, force=args.tune)
         print(f"# tunables (per-instance): {tuned_s}", file=sys.stderr, flush=True)
     solve_device(bufs[0][0], bufs[0][4], bufs[0][1], bufs[0][5], bufs[0][2], bufs[0][3])  # warmup
     check(L.gcdt_synchronize(), "sync")
@@ -388,6 +389,7 @@ def bench_batched(args):
             "speedup_cdts": round(batched_cdts / single_cdts, 2),
             "rounds": {"insert": st["insert_rounds"], "flip": st["flip_rounds"],
                        "constraints": st["constraint_rounds"]},
+            "pool_peak_bytes": pool_peak, "pool_peak_bytes_per_point": round(pool_peak / max(len(flat_p), 1), 1),
             "exactness": exact, "cpu_baseline": cpu,
             "tunables": {"batched": tuned_b["choice"] if tuned_b else "defaults",
                          "per_instance": tuned_s["choice"] if tuned_s else "defaults"}}
@@ -483,7 +485,8 @@ def bench_degenerate(args):
     try:
         if not args.no_tune:
             from gcdt_amd import tune
-            tuned = tune.load_or_tune("degenerate", lambda: solve_device(dp, n, dc, c, dt, da), repeats=3)
+            tuned = tune.load_or_tune("degenerate", lambda: solve_device(dp, n, dc, c, dt, da), repeats=3,
+                                      force=args.tune)
             print(f"# tunables (degenerate): {tuned}", file=sys.stderr, flush=True)
         solve_device(dp, n, dc, c, dt, da)
         T, st = solve_device(dp, n, dc, c, dt, da)
@@ -513,6 +516,11 @@ def bench_serving(args):
     from gcdt_amd import gen
     from gcdt_amd import serve as S
 
+    # the library's default tunables: serving crashed the host process once
+    # (SIGSEGV, no GPU fault) while the per-instance comparison's tuned values
+    # (tail_threshold 2048, grid_cap 16384) were applied; the cause is open
+    from gcdt_amd import tune
+    tune.apply(tune.DEFAULTS)
     t0 = time.time()
     raw = gen.serving_requests(count=args.serve_count, qps=args.serve_qps)
     reqs = [S.Request(i, t, gen.to_grid(p), c) for i, (t, p, c) in enumerate(raw)]
@@ -529,7 +537,7 @@ def bench_serving(args):
         r.fetch()
     summ.update({"points": int(npts), "workers": args.serve_workers, "small_n": srv.small_n,
                  "mpts_per_s": round(npts / max(st.makespan_s, 1e-9) / 1e6, 3),
-                 "latency": "arrival to triangles resident on the device"})
+                 "latency": "arrival to triangles resident on the device", "tunables": "defaults"})
     # saturation: every request waiting at t=0; continuous batching on the
     # worker streams against one plain solve at a time on one stream
     def fresh():
@@ -550,6 +558,21 @@ def bench_serving(args):
                          "p99_ms": sat.summary(0)["p99_ms"],
                          "one_at_a_time_cdts_per_s": round(one_rate, 2),
                          "speedup_cdts": round(sat_rate / max(one_rate, 1e-9), 2)}
+    # latency against load: the same requests, their Poisson arrival times
+    # rescaled to 10%, 40% and 80% of the saturated rate just measured
+    levels = []
+    for frac in (0.1, 0.4, 0.8):
+        qps = frac * sat_rate
+        lv = [S.Request(r.rid, r.arrival * args.serve_qps / qps, r.pts, r.cons) for r in reqs]
+        st_l = S.Server(workers=args.serve_workers, small_n=args.serve_small_n).run(lv)
+        for r in lv:
+            r.fetch()
+        sl = st_l.summary(round(qps, 1))
+        sl["fraction_of_saturated"] = frac
+        levels.append({k: sl[k] for k in ("fraction_of_saturated", "qps_offered", "cdts_per_s", "p50_ms", "p99_ms",
+                                          "mean_batch", "completed", "failed")})
+        print(f"# serving at {qps:.0f} QPS: {levels[-1]}", file=sys.stderr, flush=True)
+    summ["load_levels"] = levels
     refs = None
     if not args.no_cpu:
         rate, thr, secs, refs = _cpu_pool_cdts([(r.pts, r.cons) for r in reqs])

@@ -128,34 +128,45 @@ def pack_instances(instances):
 
 
 def solve_batch_device(d_pts: DeviceBuffer, poff: np.ndarray, d_cons, coff: np.ndarray, d_tris: DeviceBuffer,
-                       stream=None):
+                       stream=None, inst_status: np.ndarray | None = None):
+    """gcdt_solve_batch on device buffers.  inst_status (optional int32 array
+    of K): per-instance status, so that an instance with a bad input fails
+    alone instead of failing the call."""
     K = len(poff) - 1
     st = BatchStats()
     poff = np.ascontiguousarray(poff, dtype=np.int64)
     coff = np.ascontiguousarray(coff, dtype=np.int64)
     P64 = ctypes.POINTER(ctypes.c_int64)
+    if inst_status is not None:
+        assert inst_status.dtype == np.int32 and inst_status.flags.c_contiguous and len(inst_status) == K
     rc = lib().gcdt_solve_batch(d_pts.ptr, poff.ctypes.data_as(P64), K,
                                 d_cons.ptr if d_cons is not None else None,
                                 coff.ctypes.data_as(P64) if d_cons is not None else None,
-                                d_tris.ptr, ctypes.byref(st), stream)
+                                d_tris.ptr, ctypes.byref(st),
+                                inst_status.ctypes.data if inst_status is not None else None, stream)
     check(rc, "gcdt_solve_batch")
     return st.as_dict()
 
 
-def solve_batch(instances):
+def solve_batch(instances, isolate: bool = False):
     """K independent CDTs in one batched solve.  Returns (list of per-instance
-    triangle arrays with local vertex ids, stats)."""
+    triangle arrays with local vertex ids, stats); with isolate, stats also
+    holds "status", the per-instance GCDT_* status (an instance with a bad
+    input fails alone and its triangles are not meaningful)."""
     flat_p, poff, flat_c, coff, toff = pack_instances(instances)
     dp = DeviceBuffer.from_array(flat_p)
     dc = DeviceBuffer.from_array(flat_c) if len(flat_c) else None
     dt = DeviceBuffer(12 * int(toff[-1]))
+    status = np.zeros(len(instances), np.int32) if isolate else None
     try:
-        st = solve_batch_device(dp, poff, dc, coff, dt)
+        st = solve_batch_device(dp, poff, dc, coff, dt, inst_status=status)
         allt = dt.to_array(np.int32, (int(toff[-1]), 3))
     finally:
         for b in (dp, dc, dt):
             if b is not None:
                 b.free()
+    if isolate:
+        st["status"] = status.tolist()
     return [allt[toff[i]:toff[i + 1]] for i in range(len(instances))], st
 
 

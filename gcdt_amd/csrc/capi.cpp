@@ -131,6 +131,12 @@ int gcdt_pool_set_budget(size_t bytes)
     return GCDT_OK;
 }
 
+int gcdt_pool_reset_peak(void)
+{
+    getMemCacheRef().resetPeak();
+    return GCDT_OK;
+}
+
 int gcdt_pool_trim(void)
 {
     getMemCacheRef().releaseCache();
@@ -251,14 +257,9 @@ int gcdt_prof_names(char* buf, int cap) { return prof_names(buf, cap); }
 
 namespace gcdt {
 
-struct SolveOut {
-    int T = 0;
-    gcdt_stats st{};
-};
-
 // Full pipeline on device arrays; pts holds N+4 entries (corners written here).
-static void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, int3* tris_out,
-                       int3* adj_out, SolveOut& out, int* cflag_out = nullptr)
+void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, int3* tris_out, int3* adj_out,
+                SolveOut& out, int* cflag_out)
 {
     gcdt_stats& st = out.st;
     Timer total(c.st);
@@ -317,8 +318,8 @@ static void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, i
     DBuf<int> isv(N > 0 ? N : 1);
     fill_i32(c, isv.p, 0, N);
     int np = locate_bitmap(c, pts, N, tB.p, Ts, bo, bt, loc.p, isv.p, pend.p);
-    getMemCacheRef().releaseAbtrtMem(bo, sizeof(int) * ((1 << 20) + 1));
-    getMemCacheRef().releaseAbtrtMem(bt, sizeof(int) * (size_t)(nbt > 0 ? nbt : 1));
+    dev_give(bo, sizeof(int) * ((1 << 20) + 1));
+    dev_give(bt, sizeof(int) * (size_t)(nbt > 0 ? nbt : 1));
     isv.release();
     DBuf<int> sB(capB);
     DBuf<unsigned long long> clB(capB);
@@ -750,34 +751,17 @@ int gcdt_add_constraints(void* tmp, size_t tmpBytes, const gcdt_int2* points, gc
 
 // ------------------------------------------------------------------ batches
 
-extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int K, const gcdt_int2* cons,
-                                const int64_t* cons_offsets, gcdt_int3* tris_out, gcdt_batch_stats* stats,
-                                void* stream)
+namespace {
+
+// One batched solve of K instances; poff/coff/toff (K+1 each) start at 0.
+// Throws gcdt::Error on failure.
+void batch_impl(const gcdt_int2* points, const std::vector<long long>& poff, const gcdt_int2* cons,
+                const std::vector<long long>& coff, const std::vector<long long>& toff, int K, gcdt_int3* tris_out,
+                gcdt_batch_stats& bs, hipStream_t st)
 {
-    API_BEGIN
-    if (K <= 0 || !point_offsets || !tris_out) return fail(GCDT_ERR_INVALID, "bad arguments");
-    if (point_offsets[0] != 0) return fail(GCDT_ERR_INVALID, "point_offsets[0] must be 0");
-    std::vector<long long> poff(K + 1), coff(K + 1, 0), toff(K + 1, 0);
-    for (int i = 0; i <= K; i++) {
-        poff[i] = point_offsets[i];
-        if (i && poff[i] < poff[i - 1]) return fail(GCDT_ERR_INVALID, "point_offsets not monotone");
-        if (cons_offsets) {
-            coff[i] = cons_offsets[i];
-            if (i && coff[i] < coff[i - 1]) return fail(GCDT_ERR_INVALID, "cons_offsets not monotone");
-        }
-        if (i) toff[i] = toff[i - 1] + 2 * (poff[i] - poff[i - 1]) + 2;
-    }
-    if (cons_offsets && coff[0] != 0) return fail(GCDT_ERR_INVALID, "cons_offsets[0] must be 0");
     long long NP = poff[K], NC = coff[K];
-    if (NP > 0 && !points) return fail(GCDT_ERR_INVALID, "no points");
-    if (NC > 0 && !cons) return fail(GCDT_ERR_INVALID, "no constraints");
-    long long nv = NP + 4LL * K, cap = 2LL * K + 4 * NP + 8;
-    if (cap > (1LL << 30) || nv > (1LL << 29)) return fail(GCDT_ERR_INVALID, "batch too large");
-    // the per-instance location grid has 256 cells per instance
-    if (((long long)K << 8) > (1LL << 26)) return fail(GCDT_ERR_INVALID, "more than 262144 instances in one batch");
-    hipStream_t st = (hipStream_t)stream;
+    long long nv = NP + 4LL * K;
     Ctx c(st);
-    gcdt_batch_stats bs{};
     Timer total(c.st);
     DBuf<long long> d_poff(K + 1), d_coff(K + 1), d_toff(K + 1);
     GCDT_HIP(hipMemcpyAsync(d_poff.p, poff.data(), 8 * (K + 1), hipMemcpyHostToDevice, st));
@@ -798,7 +782,7 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
         cbase[i] = (int)ncells;
         ncells += 1LL << (2 * L);
     }
-    if (ncells > (1LL << 30)) return fail(GCDT_ERR_INVALID, "batch too large");
+    if (ncells > (1LL << 30)) throw Error(GCDT_ERR_INVALID, "batch too large");
     DBuf<int> d_lvl(K), d_cbase(K);
     GCDT_HIP(hipMemcpyAsync(d_lvl.p, lvl.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
     GCDT_HIP(hipMemcpyAsync(d_cbase.p, cbase.data(), sizeof(int) * K, hipMemcpyHostToDevice, st));
@@ -846,8 +830,8 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     DBuf<int> isv(NP > 0 ? NP : 1);
     fill_i32(c, isv.p, 0, NP);
     int np = locate_bitmap_batch(c, pts.p, vinst.p, NP, tB.p, Ts, bo, bt, loc.p, isv.p, pend.p);
-    getMemCacheRef().releaseAbtrtMem(bo, sizeof(int) * (size_t)(((long long)K << 8) + 1));
-    getMemCacheRef().releaseAbtrtMem(bt, sizeof(int) * (size_t)(nbt > 0 ? nbt : 1));
+    dev_give(bo, sizeof(int) * (size_t)(((long long)K << 8) + 1));
+    dev_give(bt, sizeof(int) * (size_t)(nbt > 0 ? nbt : 1));
     isv.release();
     DBuf<int> sB(capB);
     DBuf<unsigned long long> clB(capB);
@@ -910,7 +894,97 @@ extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_of
     bs.num_instances = K;
     bs.num_points = NP;
     bs.num_triangles = T;
+}
+
+// statuses that come from an instance's own input (bad ids or coordinates,
+// duplicate points, crossing constraints, and the consistency checks they
+// can trip), as against the device, memory or the call's arguments
+bool input_status(int s)
+{
+    return s == GCDT_ERR_INVALID || s == GCDT_ERR_NOT_CONVERGED || s == GCDT_ERR_CROSSING || s == GCDT_ERR_LOCATE ||
+           s == GCDT_ERR_CAPACITY || s == GCDT_ERR_INTERNAL;
+}
+
+// Instances [a, b) as one batch; when it fails with an input status, its two
+// halves are solved separately, down to single instances, whose status is
+// recorded.  Each instance's triangles land in its own range of tris_out.
+void batch_isolate(const gcdt_int2* points, const std::vector<long long>& poff, const gcdt_int2* cons,
+                   const std::vector<long long>& coff, const std::vector<long long>& toff, int a, int b,
+                   gcdt_int3* tris_out, gcdt_batch_stats& acc, int* inst_status, std::string& first_err,
+                   hipStream_t st)
+{
+    std::vector<long long> p(b - a + 1), c(b - a + 1), t(b - a + 1);
+    for (int i = a; i <= b; i++) {
+        p[i - a] = poff[i] - poff[a];
+        c[i - a] = coff[i] - coff[a];
+        t[i - a] = toff[i] - toff[a];
+    }
+    gcdt_batch_stats bs{};
+    try {
+        batch_impl(points + poff[a], p, cons ? cons + coff[a] : nullptr, c, t, b - a, tris_out + toff[a], bs, st);
+    } catch (const Error& e) {
+        if (!inst_status || !input_status(e.status)) throw;
+        if (st) (void)hipStreamSynchronize(st);
+        if (b - a == 1) {
+            inst_status[a] = e.status;
+            if (first_err.empty()) first_err = "instance " + std::to_string(a) + ": " + e.what();
+            return;
+        }
+        int m = a + (b - a) / 2;
+        batch_isolate(points, poff, cons, coff, toff, a, m, tris_out, acc, inst_status, first_err, st);
+        batch_isolate(points, poff, cons, coff, toff, m, b, tris_out, acc, inst_status, first_err, st);
+        return;
+    }
+    for (int i = a; i < b; i++)
+        if (inst_status) inst_status[i] = GCDT_OK;
+    acc.ms_total += bs.ms_total;
+    acc.ms_insert += bs.ms_insert;
+    acc.ms_flip += bs.ms_flip;
+    acc.ms_constraints += bs.ms_constraints;
+    acc.insert_rounds = std::max(acc.insert_rounds, bs.insert_rounds);
+    acc.flip_rounds = std::max(acc.flip_rounds, bs.flip_rounds);
+    acc.constraint_rounds = std::max(acc.constraint_rounds, bs.constraint_rounds);
+    acc.num_instances += bs.num_instances;
+    acc.num_points += bs.num_points;
+    acc.num_triangles += bs.num_triangles;
+    acc.flips += bs.flips;
+    acc.segments += bs.segments;
+}
+
+}  // namespace
+
+extern "C" int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int K, const gcdt_int2* cons,
+                                const int64_t* cons_offsets, gcdt_int3* tris_out, gcdt_batch_stats* stats,
+                                int* inst_status, void* stream)
+{
+    API_BEGIN
+    if (K <= 0 || !point_offsets || !tris_out) return fail(GCDT_ERR_INVALID, "bad arguments");
+    if (point_offsets[0] != 0) return fail(GCDT_ERR_INVALID, "point_offsets[0] must be 0");
+    std::vector<long long> poff(K + 1), coff(K + 1, 0), toff(K + 1, 0);
+    for (int i = 0; i <= K; i++) {
+        poff[i] = point_offsets[i];
+        if (i && poff[i] < poff[i - 1]) return fail(GCDT_ERR_INVALID, "point_offsets not monotone");
+        if (cons_offsets) {
+            coff[i] = cons_offsets[i];
+            if (i && coff[i] < coff[i - 1]) return fail(GCDT_ERR_INVALID, "cons_offsets not monotone");
+        }
+        if (i) toff[i] = toff[i - 1] + 2 * (poff[i] - poff[i - 1]) + 2;
+    }
+    if (cons_offsets && coff[0] != 0) return fail(GCDT_ERR_INVALID, "cons_offsets[0] must be 0");
+    long long NP = poff[K], NC = coff[K];
+    if (NP > 0 && !points) return fail(GCDT_ERR_INVALID, "no points");
+    if (NC > 0 && !cons) return fail(GCDT_ERR_INVALID, "no constraints");
+    long long nv = NP + 4LL * K, cap = 2LL * K + 4 * NP + 8;
+    if (cap > (1LL << 30) || nv > (1LL << 29)) return fail(GCDT_ERR_INVALID, "batch too large");
+    // the per-instance location grid has 256 cells per instance
+    if (((long long)K << 8) > (1LL << 26)) return fail(GCDT_ERR_INVALID, "more than 262144 instances in one batch");
+    hipStream_t st = (hipStream_t)stream;
+    gcdt_batch_stats bs{};
+    std::string first_err;
+    batch_isolate(points, poff, cons, coff, toff, 0, K, tris_out, bs, inst_status, first_err, st);
+    bs.num_instances = K;
     if (stats) *stats = bs;
+    if (!first_err.empty()) g_last_error = first_err;
     return GCDT_OK;
     API_END
 }

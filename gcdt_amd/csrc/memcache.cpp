@@ -58,6 +58,8 @@ void* MemCache::getMemByAbtrtSize(size_t size)
     void* mem = nullptr;
     int c = classOf(size);
     size_t bytes = c < 0 ? largeBytes(size) : kSizes[c];
+    // the budget bounds the bytes in use (admission control sets it)
+    if (budget_ && inUse_ + bytes > budget_) return nullptr;
     if (c >= 0 && !stkInVec[c].empty()) {
         mem = stkInVec[c].top();
         stkInVec[c].pop();

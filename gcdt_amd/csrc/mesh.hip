@@ -17,14 +17,17 @@ namespace gcdt {
 
 // ================================================================== context
 
-// Values of each tunable that a GPU parity test covers.  Anything else is
-// refused, by gcdt_set_tunable and by the environment overrides alike.
+// Values of each tunable that a GPU parity test covers (the autotuner's
+// candidates, plus tail thresholds 0 and 2^20 that force one round path).
+// Anything else is refused, by gcdt_set_tunable and by the environment
+// overrides alike.
 bool tunable_cleared(const std::string& name, int v)
 {
-    if (name == "tail_threshold") return v == 0 || v == 4096 || v == (1 << 20);
-    if (name == "grid_cap") return v == kMaxGrid;
-    if (name == "flip_items" || name == "ins_items") return v == 1;
-    if (name == "pivot" || name == "single_tester") return v == 1;
+    if (name == "tail_threshold")
+        return v == 0 || v == 1024 || v == 2048 || v == 4096 || v == 8192 || v == 16384 || v == (1 << 20);
+    if (name == "grid_cap") return v == 2048 || v == 4096 || v == 8192 || v == 16384;
+    if (name == "flip_items" || name == "ins_items") return v == 1 || v == 2 || v == 4;
+    if (name == "pivot" || name == "single_tester") return v == 0 || v == 1;
     return false;
 }
 
@@ -55,18 +58,28 @@ Tunables& Tunables::get()
 thread_local std::vector<PendingFree>* g_defer = nullptr;
 int g_grid_cap = kMaxGrid;
 
+void* dev_take(size_t bytes)
+{
+    void* p = getMemCacheRef().getMemByAbtrtSize(bytes);
+    if (!p) throw Error(GCDT_ERR_OOM, "device pool allocation failed");
+    return p;
+}
+
+void dev_give(void* p, size_t bytes)
+{
+    if (!p) return;
+    if (g_defer) g_defer->push_back({p, bytes});
+    else getMemCacheRef().releaseAbtrtMem(p, bytes);
+}
+
 namespace {
 // Pinned counter mirrors and snapshot rings, reused across solves: allocating
 // and freeing pinned memory per call costs more than a small solve and
 // hipHostFree waits for the whole device, which would serialise solves that
 // run on different streams.
-struct HostSlot {
-    int* hcnt;
-    unsigned long long* hring;
-    unsigned long long* hring_dev;
-};
 std::mutex g_host_mu;
 std::vector<HostSlot> g_host_free;
+}  // namespace
 
 HostSlot take_host_slot()
 {
@@ -88,7 +101,6 @@ HostSlot take_host_slot()
     GCDT_HIP(hipHostGetDevicePointer((void**)&h.hring_dev, h.hring, 0));
     return h;
 }
-}  // namespace
 
 Ctx::Ctx(hipStream_t s) : st(s)
 {

@@ -20,6 +20,12 @@ struct PendingFree {
 };
 extern thread_local std::vector<PendingFree>* g_defer;
 
+// device memory for a buffer from the pool
+void* dev_take(size_t bytes);
+// give back memory from dev_take: at once, or once the solve's stream has
+// drained (g_defer)
+void dev_give(void* p, size_t bytes);
+
 // Device buffer from the MemCache pool, released on destruction.
 template <class T>
 struct DBuf {
@@ -31,8 +37,7 @@ struct DBuf {
     {
         release();
         n = count;
-        p = (T*)getMemCacheRef().getMemByAbtrtSize(sizeof(T) * (count ? count : 1));
-        if (!p) throw Error(GCDT_ERR_OOM, "device pool allocation failed");
+        p = (T*)dev_take(sizeof(T) * (count ? count : 1));
     }
     void ensure(size_t count)
     {
@@ -40,13 +45,7 @@ struct DBuf {
     }
     void release()
     {
-        if (p) {
-            size_t bytes = sizeof(T) * (n ? n : 1);
-            if (g_defer)
-                g_defer->push_back({p, bytes});
-            else
-                getMemCacheRef().releaseAbtrtMem(p, bytes);
-        }
+        if (p) dev_give(p, sizeof(T) * (n ? n : 1));
         p = nullptr;
         n = 0;
     }
@@ -118,6 +117,16 @@ struct Tunables {
 };
 // whether a GPU parity test covers this value of the tunable
 bool tunable_cleared(const std::string& name, int value);
+
+// Pinned host pages of a context's counters: the mirror for synchronous
+// reads and the mapped ring of pipelined snapshots.
+struct HostSlot {
+    int* hcnt = nullptr;
+    unsigned long long* hring = nullptr;
+    unsigned long long* hring_dev = nullptr;
+};
+// a slot from the process-wide free list, or newly allocated pinned pages
+HostSlot take_host_slot();
 
 struct Ctx {
     hipStream_t st = nullptr;
@@ -297,5 +306,13 @@ ConsResult insert_constraints(Ctx& c, const Mesh& m, int T, int nv, int nmax, co
 long long mark_constraint_chains(Ctx& c, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C);
 // vertex ids n..n+3 <- the square corners (insertCornerPoints)
 void write_corners(Ctx& c, int2* pts, int n);
+
+// the whole solve (gcdt_solve / gcdt_solve_flags) on device arrays
+struct SolveOut {
+    int T = 0;
+    gcdt_stats st{};
+};
+void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, int3* tris_out, int3* adj_out,
+                SolveOut& out, int* cflag_out = nullptr);
 
 }  // namespace gcdt

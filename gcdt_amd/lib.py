@@ -71,6 +71,7 @@ SIGNATURES = {
     "gcdt_pool_init": ([], i32),
     "gcdt_pool_stats": ([ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)], i32),
     "gcdt_pool_set_budget": ([sz], i32),
+    "gcdt_pool_reset_peak": ([], i32),
     "gcdt_pool_trim": ([], i32),
     "gcdt_memcpy_h2d": ([vp, vp, sz], i32),
     "gcdt_memcpy_d2h": ([vp, vp, sz], i32),
@@ -107,7 +108,7 @@ SIGNATURES = {
     "gcdt_solve_flags": ([vp, i32, vp, i32, vp, vp, vp, pint, ctypes.POINTER(Stats), vp], i32),
     "gcdt_carve": ([vp, vp, vp, i32, i32, vp, pint, vp], i32),
     "gcdt_solve_batch": ([vp, ctypes.POINTER(ctypes.c_int64), i32, vp, ctypes.POINTER(ctypes.c_int64), vp,
-                          ctypes.POINTER(BatchStats), vp], i32),
+                          ctypes.POINTER(BatchStats), vp, vp], i32),
     "gcdt_set_tunable": ([ctypes.c_char_p, i32], i32),
     "gcdt_get_tunable": ([ctypes.c_char_p, pint], i32),
 }

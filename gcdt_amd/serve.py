@@ -5,15 +5,28 @@ time.  Worker threads, each owning one non-blocking HIP stream, repeatedly
 take whatever requests are waiting when they become free:
 
 * small requests (n <= small_n) are merged into one batched solve
-  (gcdt_solve_batch) up to `batch_points` points;
+  (gcdt_solve_batch) of up to `batch_points` points, taken in arrival order
+  from anywhere in the queue (a large request waiting ahead of them does not
+  stop the merge);
 * a large request is solved alone (gcdt_solve);
 
 so requests that arrive while others are on the GPU join the next batch of
 the first worker that frees up, and the workers' streams overlap on the
-device.  Admission is by HBM footprint: a batch starts only when its estimated
-device bytes fit in the budget next to the batches in flight (the ledger also
-records the MemCache pool's own peak).  Results stay on the device until the
-client reads them (Request.fetch()).
+device.
+
+Admission is by HBM footprint.  A batch starts only when its device bytes --
+the solve's working memory plus the triangles it leaves on the device -- fit
+next to what is in flight and what waits to be fetched, against the budget.
+The per-point figures are the MemCache pool's measured peaks for each path
+(single solve and batched solve), so the ledger never admits more than fits;
+the same budget is given to the pool (gcdt_pool_set_budget), which refuses an
+allocation past it.  Results stay on the device until the client reads them
+(Request.fetch()), which also returns their bytes to the ledger.
+
+A request that fails -- alone, or inside a merged batch, where
+gcdt_solve_batch's per-instance status isolates it -- fails alone with its
+own error; a request still waiting when its deadline passes fails with a
+timeout instead of running.
 
 The executor is pluggable: GpuExecutor drives libgcdt.so; tests substitute a
 CPU executor to exercise admission and back-pressure without a GPU.
@@ -27,18 +40,70 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-# bytes of device memory per point during a solve: the insertion DAG (4N
-# slots x 28 B) alive while the leaves are written into the packed flip
-# records (2N x 64 B), then records + final mesh (2N x 24 B) + flip lists, plus
-# per-point state and the sorted copy; a 4,096-instance batched arena peaks
-# at about 378 B per point, rounded up here; per request overhead for the
-# bitmap and scratch
-BYTES_PER_POINT = 384
-BYTES_PER_REQUEST = 16 << 20
+# Device bytes a solve needs while it runs, from the MemCache pool's measured
+# peaks: the single solve peaked at 9,666,102,272 B for 10,099,763 points
+# with 100,000 segments (957 B per point, inputs and outputs included); the
+# batched solve of the 4,096-instance workload at about 400 B per point
+# (its sample meshes and insertion DAG are per instance, the flip records
+# shared).  Per call, the fixed buffers (cell tables, bitmap, counters).
+SINGLE_BYTES_PER_POINT = 1024
+SINGLE_BYTES_PER_CALL = 64 << 20
+BATCH_BYTES_PER_POINT = 448
+BATCH_BYTES_PER_CALL = 16 << 20
+# the triangles a finished request leaves on the device until it is fetched
+RESULT_BYTES_PER_POINT = 24
+
+
+def result_bytes(n: int) -> int:
+    return 12 * (2 * max(n, 0) + 2)
 
 
 def footprint(n: int) -> int:
-    return BYTES_PER_POINT * max(n, 1) + BYTES_PER_REQUEST
+    """Device bytes of one request solved alone: working memory + result."""
+    return SINGLE_BYTES_PER_POINT * max(n, 1) + SINGLE_BYTES_PER_CALL + result_bytes(n)
+
+
+def batch_footprint(ns) -> int:
+    """Device bytes of one merged batch of requests with these point counts."""
+    return BATCH_BYTES_PER_CALL + sum(BATCH_BYTES_PER_POINT * max(n, 1) + result_bytes(n) for n in ns)
+
+
+class DeviceResult:
+    """A request's triangles on the device: a range of a batch's output
+    buffer, shared by the batch's requests; the buffer is freed when the
+    last of them has been read or dropped."""
+
+    def __init__(self, buf, offset_tris: int, count: int, owner):
+        self.buf = buf
+        self.offset = offset_tris
+        self.count = count
+        self.owner = owner  # _SharedBuffer
+
+    def to_array(self) -> np.ndarray:
+        from .lib import check, lib
+        out = np.empty((self.count, 3), np.int32)
+        if out.nbytes:
+            check(lib().gcdt_memcpy_d2h(out.ctypes.data, self.buf.ptr + 12 * self.offset, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.owner is not None:
+            self.owner.drop()
+            self.owner = None
+
+
+class _SharedBuffer:
+    def __init__(self, buf, refs: int):
+        self.buf = buf
+        self.refs = refs
+        self.lock = threading.Lock()
+
+    def drop(self):
+        with self.lock:
+            self.refs -= 1
+            last = self.refs == 0
+        if last:
+            self.buf.free()
 
 
 @dataclass
@@ -47,12 +112,15 @@ class Request:
     arrival: float                 # seconds after the stream start
     pts: np.ndarray                # (n,2) int32 grid coordinates
     cons: np.ndarray               # (c,2) int32 local vertex ids
+    deadline: float | None = None  # seconds after arrival to be admitted by, or fail
     t_admit: float = -1.0
     t_done: float = -1.0
     batch_size: int = 0
     error: str | None = None
     result: np.ndarray | None = None   # host copy once fetched
-    _dev: object = None                # device triangle buffer (GPU executor)
+    _dev: object = None                # DeviceResult (GPU executor)
+    _ledger: object = None             # returns the result's bytes on fetch
+    _held: int = 0
 
     @property
     def n(self) -> int:
@@ -62,16 +130,28 @@ class Request:
     def latency(self) -> float:
         return self.t_done - self.arrival
 
-    def fetch(self) -> np.ndarray:
+    def fetch(self) -> np.ndarray | None:
+        """The triangles (host copy), read from the device on first call; the
+        device range and its ledger bytes are released then."""
         if self.result is None and self._dev is not None:
-            self.result = self._dev.to_array(np.int32, (2 * self.n + 2, 3))
-            self._dev.free()
-            self._dev = None
+            try:
+                self.result = self._dev.to_array()
+            finally:
+                self._dev.free()
+                self._dev = None
+        self._release()
         return self.result
+
+    def _release(self):
+        if self._ledger is not None and self._held:
+            self._ledger.release(self._held)
+        self._held = 0
+        self._ledger = None
 
 
 class Ledger:
-    """HBM-footprint admission: reserve before a batch starts, release after."""
+    """HBM-footprint admission: reserve before a batch starts; the working
+    part is released when the batch ends, the result part at fetch."""
 
     def __init__(self, budget: int):
         self.budget = int(budget)
@@ -92,8 +172,13 @@ class Ledger:
             self.cv.notify_all()
 
 
+STATUS_TEXT = {1: "out of device memory", 2: "invalid input", 3: "not converged", 5: "crossing constraints",
+               7: "duplicate point or location failure", 8: "triangle capacity exceeded", 9: "internal check failed"}
+
+
 class GpuExecutor:
-    """Runs one batch on its own stream through the C ABI."""
+    """Runs one batch on its own stream through the C ABI; returns one error
+    (None for success) per request."""
 
     def __init__(self):
         import ctypes
@@ -125,12 +210,15 @@ class GpuExecutor:
             try:
                 solve_device(dp, r.n, dc, len(r.cons), dt, None, stream=self.stream)
                 self._check(L.gcdt_stream_synchronize(self.stream), "sync")
+            except Exception:
+                dt.free()
+                raise
             finally:
                 dp.free()
                 if dc is not None:
                     dc.free()
-            r._dev = dt
-            return
+            r._dev = DeviceResult(dt, 0, 2 * r.n + 2, _SharedBuffer(dt, 1))
+            return [None]
         flat_p, poff, flat_c, coff, toff = pack_instances([(r.pts, r.cons) for r in batch])
         dp = DeviceBuffer(flat_p.nbytes)
         self._check(L.gcdt_memcpy_h2d_async(dp.ptr, flat_p.ctypes.data, flat_p.nbytes, self.stream), "h2d")
@@ -139,23 +227,37 @@ class GpuExecutor:
             dc = DeviceBuffer(flat_c.nbytes)
             self._check(L.gcdt_memcpy_h2d_async(dc.ptr, flat_c.ctypes.data, flat_c.nbytes, self.stream), "h2d")
         dt = DeviceBuffer(12 * int(toff[-1]))
+        status = np.zeros(len(batch), np.int32)
         try:
-            solve_batch_device(dp, poff, dc, coff, dt, stream=self.stream)
-            allt = np.empty((int(toff[-1]), 3), np.int32)
-            self._check(L.gcdt_memcpy_d2h_async(allt.ctypes.data, dt.ptr, allt.nbytes, self.stream), "d2h")
+            solve_batch_device(dp, poff, dc, coff, dt, stream=self.stream, inst_status=status)
             self._check(L.gcdt_stream_synchronize(self.stream), "sync")
+        except Exception:
+            dt.free()
+            raise
         finally:
-            for b in (dp, dc, dt):
+            for b in (dp, dc):
                 if b is not None:
                     b.free()
+        ok = [i for i in range(len(batch)) if status[i] == 0]
+        if not ok:
+            dt.free()
+        shared = _SharedBuffer(dt, len(ok)) if ok else None
+        errors = []
         for i, r in enumerate(batch):
-            r.result = allt[toff[i]:toff[i + 1]]
+            if status[i] == 0:
+                r._dev = DeviceResult(dt, int(toff[i]), int(toff[i + 1] - toff[i]), shared)
+                errors.append(None)
+            else:
+                errors.append(f"gcdt_solve_batch: {STATUS_TEXT.get(int(status[i]), int(status[i]))} "
+                              f"(status {int(status[i])})")
+        return errors
 
 
 @dataclass
 class ServeStats:
     completed: int = 0
     failed: int = 0
+    timed_out: int = 0
     makespan_s: float = 0.0
     latencies_s: list = field(default_factory=list)
     batches: int = 0
@@ -165,7 +267,8 @@ class ServeStats:
     def summary(self, qps: float):
         lat = np.array(self.latencies_s) * 1000.0 if self.latencies_s else np.zeros(1)
         return {"qps_offered": qps, "requests": self.completed + self.failed, "completed": self.completed,
-                "failed": self.failed, "cdts_per_s": round(self.completed / max(self.makespan_s, 1e-9), 2),
+                "failed": self.failed, "timed_out": self.timed_out,
+                "cdts_per_s": round(self.completed / max(self.makespan_s, 1e-9), 2),
                 "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
                 "mean_ms": round(float(lat.mean()), 3), "batches": self.batches,
                 "mean_batch": round(float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.0, 2),
@@ -174,7 +277,10 @@ class ServeStats:
 
 class Server:
     def __init__(self, executor_factory=GpuExecutor, workers: int = 3, hbm_budget: int = 64 << 30,
-                 small_n: int = 65536, batch_points: int = 2_000_000, clock=time.perf_counter):
+                 small_n: int = 65536, batch_points: int = 2_000_000, clock=time.perf_counter,
+                 pool_budget: bool | None = None):
+        """pool_budget: also give hbm_budget to the device pool, which then
+        refuses allocations past it (default: with the GPU executor)."""
         self.executor_factory = executor_factory
         self.workers = workers
         self.ledger = Ledger(hbm_budget)
@@ -186,11 +292,34 @@ class Server:
         self.closed = False
         self.stats = ServeStats()
         self.t0 = 0.0
+        if pool_budget is None:
+            pool_budget = executor_factory is GpuExecutor
+        self.pool_budget = pool_budget
 
     # ---- batching policy (pure; unit-tested on the CPU)
+    def _expire(self):
+        """Requests whose deadline passed while waiting fail with a timeout.
+        Caller holds the lock."""
+        now = self.now()
+        keep = collections.deque()
+        for r in self.waiting:
+            if r.deadline is not None and now - r.arrival > r.deadline:
+                r.error = f"timeout: not admitted within {r.deadline:.3f} s"
+                r.t_done = now
+                self.stats.failed += 1
+                self.stats.timed_out += 1
+            else:
+                keep.append(r)
+        self.waiting = keep
+
     def take_batch(self):
-        """Pop the next batch from `waiting` (FIFO) if its footprint fits the
-        ledger; returns (batch, bytes) or (None, 0).  Caller holds the lock."""
+        """Pop the next batch from `waiting` if its footprint fits the ledger;
+        returns (batch, bytes) or (None, 0).  Caller holds the lock.
+
+        The oldest request decides: a large one runs alone; a small one is
+        merged with the next small requests in arrival order, from anywhere in
+        the queue, up to batch_points points."""
+        self._expire()
         if not self.waiting:
             return None, 0
         head = self.waiting[0]
@@ -200,20 +329,21 @@ class Server:
                 return None, 0
             self.waiting.popleft()
             return [head], need
-        batch, pts, need = [], 0, BYTES_PER_REQUEST
+        batch, pts = [], 0
+        rest = collections.deque()
         while self.waiting:
-            r = self.waiting[0]
-            if r.n > self.small_n or (batch and pts + r.n > self.batch_points):
-                break
-            extra = BYTES_PER_POINT * max(r.n, 1)
-            if batch and not self.ledger.fits(need + extra):
-                break
-            batch.append(self.waiting.popleft())
+            r = self.waiting.popleft()
+            if r.n > self.small_n or (batch and pts + r.n > self.batch_points) or (
+                    batch and not self.ledger.fits(batch_footprint([q.n for q in batch] + [r.n]))):
+                rest.append(r)
+                continue
+            batch.append(r)
             pts += r.n
-            need += extra
-        if not self.ledger.fits(need) and self.ledger.used > 0:
-            for r in reversed(batch):
-                self.waiting.appendleft(r)
+        self.waiting = rest
+        need = batch_footprint([r.n for r in batch])
+        if not self.ledger.fits(need):
+            # put them back in arrival order
+            self.waiting = collections.deque(sorted(list(self.waiting) + batch, key=lambda q: (q.arrival, q.rid)))
             return None, 0
         return batch, need
 
@@ -232,29 +362,35 @@ class Server:
                             break
                         if self.closed and not self.waiting:
                             return
-                        self.cv.wait(timeout=0.05)
+                        self.cv.wait(timeout=0.005)
                 t = self.now()
                 for r in batch:
                     r.t_admit = t
                     r.batch_size = len(batch)
                 try:
-                    ex.run(batch)
-                    err = None
+                    errors = ex.run(batch)
+                    if errors is None:
+                        errors = [None] * len(batch)
                 except Exception as e:  # one failing batch does not stop the server
-                    err = str(e)
+                    errors = [str(e)] * len(batch)
                 t = self.now()
+                held = 0
                 with self.cv:
-                    for r in batch:
+                    for r, err in zip(batch, errors):
                         r.t_done = t
                         r.error = err
                         if err is None:
                             self.stats.completed += 1
                             self.stats.latencies_s.append(r.latency)
+                            # the result stays reserved until it is fetched
+                            r._held = result_bytes(r.n)
+                            r._ledger = self.ledger
+                            held += r._held
                         else:
                             self.stats.failed += 1
                     self.stats.batches += 1
                     self.stats.batch_sizes.append(len(batch))
-                self.ledger.release(need)
+                self.ledger.release(need - held)
         finally:
             if hasattr(ex, "close"):
                 ex.close()
@@ -264,6 +400,20 @@ class Server:
         them to completion; with realtime=False all of them are waiting at
         the start.  Returns the stats object."""
         reqs = sorted(requests, key=lambda r: r.arrival)
+        if self.pool_budget:
+            from .lib import check, lib
+            check(lib().gcdt_pool_set_budget(self.ledger.budget), "gcdt_pool_set_budget")
+        try:
+            self._serve(reqs, realtime)
+        finally:
+            if self.pool_budget:
+                from .lib import lib
+                lib().gcdt_pool_set_budget(0)
+        self.stats.makespan_s = max((r.t_done for r in reqs), default=0.0) - min((r.arrival for r in reqs), default=0.0)
+        self.stats.peak_reserved = self.ledger.peak
+        return self.stats
+
+    def _serve(self, reqs, realtime):
         self.t0 = self.clock()
         threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(self.workers)]
         if not realtime:  # everything is waiting before the first worker looks
@@ -283,6 +433,3 @@ class Server:
             self.cv.notify_all()
         for th in threads:
             th.join()
-        self.stats.makespan_s = max((r.t_done for r in reqs), default=0.0) - min((r.arrival for r in reqs), default=0.0)
-        self.stats.peak_reserved = self.ledger.peak
-        return self.stats

@@ -41,8 +41,8 @@ DEFAULTS = {"tail_threshold": 4096, "grid_cap": 4096, "flip_items": 1, "ins_item
 # Values the library accepts: those a GPU parity test covers
 # (gcdt_amd/csrc/mesh.hip tunable_cleared).  A sweep over the others is
 # refused by gcdt_set_tunable with GCDT_ERR_INVALID.
-CLEARED = {"tail_threshold": [0, 4096, 1 << 20], "grid_cap": [4096], "flip_items": [1], "ins_items": [1],
-           "pivot": [1], "single_tester": [1]}
+CLEARED = {"tail_threshold": [0, 1024, 2048, 4096, 8192, 16384, 1 << 20], "grid_cap": [2048, 4096, 8192, 16384],
+           "flip_items": [1, 2, 4], "ins_items": [1, 2, 4], "pivot": [0, 1], "single_tester": [0, 1]}
 
 
 def cache_path() -> str:

@@ -11,7 +11,8 @@
 //     preallocates about 3.1 GiB, twice; MemCache.cpp:60-76,106-116);
 //   - a mutex makes the pool usable from several host threads;
 //   - a byte ledger (bytesInUse / bytesCached / peak) and an optional budget
-//     that admission control of the batched/serving paths reads;
+//     on the bytes in use, which the serving path sets from its admission
+//     budget;
 //   - requests above 512 MiB are rounded up to 64 MiB and cached like the
 //     classes (the reference cudaMalloc/cudaFrees them every time, and every
 //     free synchronises the device); when hipMalloc fails the whole cache is
@@ -44,7 +45,8 @@ public:
     size_t bytesCached() const { return cached_; }
     size_t peakBytes() const { return peak_; }
     void resetPeak() { peak_ = inUse_; }
-    // budget for admission control (0 = unlimited)
+    // budget for admission control (0 = unlimited): an allocation that would
+    // take the bytes in use past it fails (returns nullptr)
     void setBudget(size_t bytes) { budget_ = bytes; }
     size_t budget() const { return budget_; }
     // bytes a request of `size` occupies in the pool

@@ -61,7 +61,11 @@ int gcdt_alloc(void** ptr, size_t bytes);
 int gcdt_release(void* ptr, size_t bytes);
 int gcdt_pool_init(void);                 /* MemCache::initAbtrtMem */
 int gcdt_pool_stats(size_t* in_use, size_t* cached, size_t* peak);
+/* Bytes in use that the pool will not exceed: an allocation past the budget
+ * fails with GCDT_ERR_OOM (0 = no budget, the default). */
 int gcdt_pool_set_budget(size_t bytes);
+/* Restarts the peak that gcdt_pool_stats reports at the bytes now in use. */
+int gcdt_pool_reset_peak(void);
 /* Returns every cached (free) pool block to the device; blocks in use stay. */
 int gcdt_pool_trim(void);
 int gcdt_memcpy_h2d(void* dst, const void* host_src, size_t bytes);
@@ -236,9 +240,17 @@ typedef struct {
  * so a batch costs about as many rounds as its largest instance.  Like
  * gcdt_solve, each instance first triangulates a sample (one point per cell
  * of its own 16x16 grid), makes it Delaunay and locates the other points
- * through a per-instance bitmap of it.  K <= 262144. */
+ * through a per-instance bitmap of it.  K <= 262144.
+ * inst_status (optional, HOST array of K ints): with it, an instance whose
+ * own input makes the solve fail (bad ids or coordinates, duplicate points,
+ * crossing constraints) fails alone: the batch is split in halves down to
+ * that instance, inst_status[i] receives each instance's GCDT_* status, the
+ * other instances' triangles are written as usual, the call returns GCDT_OK
+ * and gcdt_last_error() names the first failed instance.  Without it, or
+ * for device and memory errors, the whole call fails. */
 int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int K, const gcdt_int2* cons,
-                     const int64_t* cons_offsets, gcdt_int3* tris_out, gcdt_batch_stats* stats, void* stream);
+                     const int64_t* cons_offsets, gcdt_int3* tris_out, gcdt_batch_stats* stats, int* inst_status,
+                     void* stream);
 
 /* Process-wide tunables read by every later solve (set them between solves,
  * not while a solve runs on another thread): "tail_threshold" [0, 2^20] (list

@@ -102,7 +102,8 @@ class Buf:
             self.ptr = None
 
 
-def solve(g, cons):
+def solve(g, cons, entry="gcdt_solve"):
+    """entry: the name of a solve entry point with gcdt_solve's arguments."""
     g = np.ascontiguousarray(g, np.int32).reshape(-1, 2)
     c = np.ascontiguousarray(cons, np.int32).reshape(-1, 2)
     n = len(g)
@@ -110,8 +111,8 @@ def solve(g, cons):
     dc = Buf.of(c) if len(c) else None
     nt = ctypes.c_int(0)
     st = L.Stats()
-    check(load().gcdt_solve(dp.ptr, n, dc.ptr if dc else None, len(c), dt.ptr, da.ptr, ctypes.byref(nt),
-                            ctypes.byref(st), None), "gcdt_solve")
+    check(getattr(load(), entry)(dp.ptr, n, dc.ptr if dc else None, len(c), dt.ptr, da.ptr, ctypes.byref(nt),
+                                 ctypes.byref(st), None), entry)
     tris = dt.get(np.int32, (nt.value, 3))
     adj = da.get(np.int32, (nt.value, 3))
     for b in (dp, dt, da, dc):
@@ -129,7 +130,7 @@ def solve_batch(instances):
     st = L.BatchStats()
     P64 = ctypes.POINTER(ctypes.c_int64)
     check(load().gcdt_solve_batch(dp.ptr, poff.ctypes.data_as(P64), len(instances), dc.ptr if dc else None,
-                                  coff.ctypes.data_as(P64) if dc else None, dt.ptr, ctypes.byref(st), None),
+                                  coff.ctypes.data_as(P64) if dc else None, dt.ptr, ctypes.byref(st), None, None),
           "gcdt_solve_batch")
     allt = dt.get(np.int32, (int(toff[-1]), 3))
     for b in (dp, dt, dc):

@@ -96,3 +96,4 @@ def test_detector_catches_a_wrong_size_release():
     code = "from tests.emu import emu; L = emu.load(); L.emu_selftest_release()"
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
     assert r.returncode == 91, (r.returncode, r.stderr)
+

@@ -73,3 +73,33 @@ def test_batch_rejects_cross_instance_constraint():
     bad = np.array([[0, 5]], np.int32)  # id 5 does not exist in a 3-point instance
     with pytest.raises(GcdtError):
         solve_batch([(a, bad), (a, np.zeros((0, 2), np.int32))])
+
+
+def _crossing_instance():
+    """Two constraints crossing at a point that is not a vertex."""
+    rng = np.random.default_rng(1)
+    p = np.concatenate([rng.integers(0, 1 << 20, size=(300, 2)),
+                        [[1000, 1000], [900000, 900000], [1000, 900000], [900000, 1000]]])
+    p, _ = gen.dedup(p)
+    idx = {tuple(x): i for i, x in enumerate(p.tolist())}
+    a, b, c, d = idx[(1000, 1000)], idx[(900000, 900000)], idx[(1000, 900000)], idx[(900000, 1000)]
+    return gen.to_grid(p), np.array([[a, b], [c, d]], np.int32)
+
+
+def test_batch_isolates_a_failing_instance():
+    """With per-instance status, an instance whose constraints cross fails
+    alone (GCDT_ERR_CROSSING) and every other instance is still solved."""
+    from gcdt_amd import solve_batch
+    inst = _instances(9, seed=5)
+    inst.insert(4, _crossing_instance())
+    res, st = solve_batch(inst, isolate=True)
+    assert st["status"][4] == 5
+    assert [s for i, s in enumerate(st["status"]) if i != 4] == [0] * (len(inst) - 1)
+    for i, ((g, c), tris) in enumerate(zip(inst, res)):
+        if i != 4:
+            assert O.validate(g, tris, c)["valid"], i
+    # without per-instance status the whole call fails
+    from gcdt_amd import GcdtError
+    with pytest.raises(GcdtError) as e:
+        solve_batch(inst)
+    assert e.value.status == 5

@@ -47,3 +47,27 @@ def test_concurrent_streams_do_not_share_live_blocks():
     for r in reqs:
         v = O.validate(r.pts, r.fetch(), r.cons)
         assert v["valid"], (r.rid, v)
+
+
+def test_a_crossing_request_fails_alone_in_a_merged_batch():
+    """One request with crossing constraints merged with others: it fails
+    with its own error, the others complete, equal to their solves alone,
+    and their triangles stay on the device until fetched."""
+    from tests.test_gpu_batch import _crossing_instance
+    reqs = [S.Request(i, 0.0, gen.to_grid(p), c)
+            for i, (t, p, c) in enumerate(gen.serving_requests(count=10, qps=100.0, nmin=200, nmax=20_000, seed=17))]
+    g, c = _crossing_instance()
+    reqs.insert(5, S.Request(99, 0.0, g, c))
+    st = S.Server(workers=1, small_n=100_000).run(reqs, realtime=False)
+    assert st.batches == 1 and st.batch_sizes == [len(reqs)]
+    assert st.failed == 1 and st.completed == len(reqs) - 1
+    assert "crossing" in reqs[5].error
+    for r in reqs:
+        if r.rid == 99:
+            continue
+        assert r.error is None and r._dev is not None and r.result is None  # still on the device
+        tris = r.fetch()
+        assert O.validate(r.pts, tris, r.cons)["valid"]
+        alone, _ = solve(r.pts, r.cons)
+        if O.validate(r.pts, alone, r.cons)["unique"]:
+            assert np.array_equal(O.canonical(tris), O.canonical(alone))

@@ -17,20 +17,51 @@ def _req(rid, n, t=0.0):
     return S.Request(rid, t, np.zeros((n, 2), np.int32), np.zeros((0, 2), np.int32))
 
 
-def test_take_batch_groups_small_requests_fifo():
+def test_take_batch_merges_small_requests_in_arrival_order():
     srv = S.Server(executor_factory=None, small_n=100, batch_points=250)
     for i, n in enumerate([50, 60, 70, 80, 500, 10]):
-        srv.waiting.append(_req(i, n))
+        srv.waiting.append(_req(i, n, 0.001 * i))
     b, need = srv.take_batch()
-    assert [r.rid for r in b] == [0, 1, 2]          # 180 points; adding 80 would pass 250
-    assert need == S.BYTES_PER_REQUEST + S.BYTES_PER_POINT * 180
+    # 180 points; 80 would pass 250 and 500 is large, but 10 still fits
+    assert [r.rid for r in b] == [0, 1, 2, 5]
+    assert need == S.batch_footprint([50, 60, 70, 10])
     b, _ = srv.take_batch()
-    assert [r.rid for r in b] == [3]                # stops at the large request (FIFO)
+    assert [r.rid for r in b] == [3]                # the large request behind it waits its turn
     b, need = srv.take_batch()
     assert [r.rid for r in b] == [4] and need == S.footprint(500)
-    b, _ = srv.take_batch()
-    assert [r.rid for r in b] == [5]
     assert srv.take_batch() == (None, 0)
+
+
+def test_footprints_cover_the_measured_pool_peaks():
+    """The per-path figures are at least the MemCache pool's measured peaks:
+    the single solve of the 10M workload (10,099,763 points, 100,000 segments)
+    peaked at 9,666,102,272 bytes in use."""
+    assert S.footprint(10_099_763) >= 9_666_102_272
+    # a batch never counts less than its results plus the solve's own arena
+    ns = [1000, 65536, 64]
+    assert S.batch_footprint(ns) >= sum(S.result_bytes(n) for n in ns) + 378 * sum(ns)
+
+
+def test_results_hold_their_bytes_until_fetched():
+    reqs = [_req(i, 1000 + i, 0.0) for i in range(4)]
+    srv = S.Server(executor_factory=SleepExecutor, workers=2, small_n=10)
+    srv.run(reqs, realtime=False)
+    held = sum(S.result_bytes(r.n) for r in reqs)
+    assert srv.ledger.used == held
+    for r in reqs:
+        r.fetch()
+    assert srv.ledger.used == 0
+    reqs[0].fetch()                                 # a second fetch releases nothing more
+    assert srv.ledger.used == 0
+
+
+def test_waiting_past_the_deadline_fails_with_a_timeout():
+    reqs = [_req(i, 50_000, 0.0) for i in range(3)]
+    reqs[2].deadline = 0.0005                       # needs admission within 0.5 ms
+    srv = S.Server(executor_factory=SleepExecutor, workers=1, small_n=10)
+    st = srv.run(reqs, realtime=False)
+    assert st.completed == 2 and st.failed == 1 and st.timed_out == 1
+    assert reqs[2].error.startswith("timeout") and reqs[2].t_admit < 0
 
 
 def test_admission_holds_back_when_budget_is_full():
@@ -77,7 +108,7 @@ def test_server_completes_every_request_within_budget():
     rng = np.random.default_rng(3)
     reqs = [_req(i, int(n), t) for i, (n, t) in
             enumerate(zip(rng.integers(10, 200_000, 80), np.cumsum(rng.exponential(0.002, 80))))]
-    budget = S.footprint(400_000)
+    budget = S.footprint(400_000) + sum(S.result_bytes(r.n) for r in reqs)
     srv = S.Server(executor_factory=SleepExecutor, workers=3, hbm_budget=budget, small_n=50_000,
                    batch_points=150_000)
     st = srv.run(reqs)
@@ -106,6 +137,33 @@ def test_a_failing_batch_does_not_stop_the_server():
     st = srv.run(reqs, realtime=False)
     assert st.failed == 1 and st.completed == 5
     assert reqs[2].error == "device error" and reqs[3].error is None
+
+
+class PerRequestExecutor:
+    """Like the GPU executor with per-instance status: a bad request inside a
+    merged batch fails alone."""
+
+    def run(self, batch):
+        errors = []
+        for r in batch:
+            if r.rid == 3:
+                errors.append("gcdt_solve_batch: crossing constraints (status 5)")
+            else:
+                r.result = np.full((2 * r.n + 2, 3), r.rid, np.int32)
+                errors.append(None)
+        return errors
+
+
+def test_a_bad_request_in_a_merged_batch_fails_alone():
+    reqs = [_req(i, 100 + i, 0.0) for i in range(8)]
+    srv = S.Server(executor_factory=PerRequestExecutor, workers=1, small_n=1000)
+    st = srv.run(reqs, realtime=False)
+    assert max(st.batch_sizes) == 8                 # all of them in one batch
+    assert st.failed == 1 and st.completed == 7
+    assert "crossing" in reqs[3].error
+    for r in reqs:
+        if r.rid != 3:
+            assert r.error is None and np.all(r.fetch() == r.rid)
 
 
 class OracleExecutor:

@@ -57,8 +57,9 @@ def test_tunable_ranges_are_checked():
 
 
 def test_only_gpu_tested_values_are_accepted():
-    """Values without a passing GPU parity test are refused; the covered
-    ones (the defaults and the two extreme tail thresholds) are accepted."""
+    """Values without a passing GPU parity test (tests/test_gpu_tunables.py
+    runs every candidate) are refused; the covered ones are accepted, and
+    every candidate the tuner tries is covered."""
     saved = {k: get_tunable(k) for k in tune.DEFAULTS}
     try:
         for k, v in tune.CLEARED.items():
@@ -66,10 +67,10 @@ def test_only_gpu_tested_values_are_accepted():
                 set_tunable(k, x)
                 assert get_tunable(k) == x
         for k, v in tune.CANDIDATES.items():
-            for x in v:
-                if x not in tune.CLEARED[k]:
-                    with pytest.raises(GcdtError, match="not covered"):
-                        set_tunable(k, x)
+            assert set(v) <= set(tune.CLEARED[k]), k
+        for k, x in (("tail_threshold", 3000), ("grid_cap", 1000), ("grid_cap", 65536), ("tail_threshold", 64)):
+            with pytest.raises(GcdtError, match="not covered"):
+                set_tunable(k, x)
     finally:
         for k, v in saved.items():
             set_tunable(k, v)
