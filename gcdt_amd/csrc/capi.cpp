@@ -34,14 +34,25 @@ int fail(int status, const std::string& msg)
 struct Timer {
     hipEvent_t a = nullptr, b = nullptr;
     hipStream_t st;
+    // inside an arena call no events are created: the host clock from here
+    // to the stream's drain in ms()
+    std::chrono::steady_clock::time_point h0;
     explicit Timer(hipStream_t s) : st(s)
     {
+        if (g_arena) {
+            h0 = std::chrono::steady_clock::now();
+            return;
+        }
         GCDT_HIP(hipEventCreate(&a));
         GCDT_HIP(hipEventCreate(&b));
         GCDT_HIP(hipEventRecord(a, st));
     }
     double ms()
     {
+        if (!a) {
+            GCDT_HIP(hipStreamSynchronize(st));
+            return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+        }
         GCDT_HIP(hipEventRecord(b, st));
         GCDT_HIP(hipEventSynchronize(b));
         float f = 0;
@@ -112,6 +123,7 @@ int gcdt_pool_init(void)
 {
     API_BEGIN
     getMemCacheRef().initAbtrtMem();
+    noalloc_setup();
     return GCDT_OK;
     API_END
 }

@@ -58,8 +58,48 @@ Tunables& Tunables::get()
 thread_local std::vector<PendingFree>* g_defer = nullptr;
 int g_grid_cap = kMaxGrid;
 
+thread_local Arena* g_arena = nullptr;
+
+void* Arena::take(size_t bytes)
+{
+    size_t len = (bytes + 255) & ~(size_t)255;
+    if (len > cap - top)
+        throw Error(GCDT_ERR_SCRATCH, "the scratch array (" + std::to_string(cap) + " bytes) is too small for this "
+                                      "input: " + std::to_string(top + len) + " bytes needed so far");
+    void* p = base + top;
+    top += len;
+    if (top > high) high = top;
+    return p;
+}
+
+void Arena::give(void* p, size_t bytes)
+{
+    size_t off = (size_t)((char*)p - base), end = off + ((bytes + 255) & ~(size_t)255);
+    if (end != top) {
+        if (nfreed < kFreed) {
+            freed_off[nfreed] = off;
+            freed_end[nfreed] = end;
+            nfreed++;
+        }
+        return;  // a full list leaves the block in use until the call ends
+    }
+    top = off;
+    for (bool again = true; again;) {
+        again = false;
+        for (int i = 0; i < nfreed; i++)
+            if (freed_end[i] == top) {
+                top = freed_off[i];
+                freed_off[i] = freed_off[--nfreed];
+                freed_end[i] = freed_end[nfreed];
+                again = true;
+                break;
+            }
+    }
+}
+
 void* dev_take(size_t bytes)
 {
+    if (g_arena) return g_arena->take(bytes);
     void* p = getMemCacheRef().getMemByAbtrtSize(bytes);
     if (!p) throw Error(GCDT_ERR_OOM, "device pool allocation failed");
     return p;
@@ -68,7 +108,8 @@ void* dev_take(size_t bytes)
 void dev_give(void* p, size_t bytes)
 {
     if (!p) return;
-    if (g_defer) g_defer->push_back({p, bytes});
+    if (g_arena && g_arena->owns(p)) g_arena->give(p, bytes);
+    else if (g_defer) g_defer->push_back({p, bytes});
     else getMemCacheRef().releaseAbtrtMem(p, bytes);
 }
 
@@ -102,7 +143,7 @@ HostSlot take_host_slot()
     return h;
 }
 
-Ctx::Ctx(hipStream_t s) : st(s)
+Ctx::Ctx(hipStream_t s, const HostSlot* fixed) : st(s)
 {
     if (!Tunables::get().env_error.empty()) throw Error(GCDT_ERR_INVALID, Tunables::get().env_error);
     tail_threshold = Tunables::get().tail_threshold;
@@ -111,7 +152,8 @@ Ctx::Ctx(hipStream_t s) : st(s)
     flip_items = Tunables::get().flip_items;
     ins_items = Tunables::get().ins_items;
     dcnt.alloc(C_NUM);
-    HostSlot h = take_host_slot();
+    HostSlot h = fixed ? *fixed : take_host_slot();
+    fixed_slot = fixed != nullptr;
     hcnt = h.hcnt;
     hring = h.hring;
     hring_dev = h.hring_dev;
@@ -131,7 +173,7 @@ Ctx::~Ctx()
         for (auto& f : deferred) getMemCacheRef().releaseAbtrtMem(f.p, f.bytes);
         deferred.clear();
     }
-    if (hcnt) {
+    if (hcnt && !fixed_slot) {
         std::lock_guard<std::mutex> g(g_host_mu);
         g_host_free.push_back(HostSlot{hcnt, hring, hring_dev});
     }
@@ -258,7 +300,8 @@ bool take_event(hipEvent_t* e)
 
 ProfScope::ProfScope(const char* name, hipStream_t s, long long items) : st(s)
 {
-    if (!g_prof_on) return;
+    // no events (and no bookkeeping that allocates) inside an arena call
+    if (!g_prof_on || g_arena) return;
     std::lock_guard<std::mutex> g(g_prof_mu);
     if (!g_prof_only.empty() && g_prof_only != name) return;
     for (size_t i = 0; i < g_prof.size(); i++)

@@ -20,13 +20,36 @@ struct PendingFree {
 };
 extern thread_local std::vector<PendingFree>* g_defer;
 
-// device memory for a buffer from the pool
+// A region of device memory that this thread's device buffers come from
+// instead of the pool while it is set (g_arena): the allocation-free solve
+// carves every buffer out of a fixed scratch array.  Buffers are taken from
+// the top, 256-byte aligned; a released buffer is given back when it is the
+// top one (with any released buffers it uncovers), otherwise it is remembered
+// and given back once the buffers above it are.  Taking more than the region
+// holds throws GCDT_ERR_SCRATCH.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t top = 0;
+    size_t high = 0;  // most bytes in use at once
+    static constexpr int kFreed = 64;
+    size_t freed_off[kFreed];
+    size_t freed_end[kFreed];
+    int nfreed = 0;
+    Arena(void* b, size_t c) : base((char*)b), cap(c) {}
+    bool owns(const void* p) const { return (const char*)p >= base && (const char*)p < base + cap; }
+    void* take(size_t bytes);
+    void give(void* p, size_t bytes);
+};
+extern thread_local Arena* g_arena;
+
+// device memory for a buffer: the arena when one is set, else the pool
 void* dev_take(size_t bytes);
-// give back memory from dev_take: at once, or once the solve's stream has
-// drained (g_defer)
+// give back memory from dev_take (or from the pool: the same rule)
 void dev_give(void* p, size_t bytes);
 
-// Device buffer from the MemCache pool, released on destruction.
+// Device buffer from the MemCache pool (or the thread's arena), released on
+// destruction.
 template <class T>
 struct DBuf {
     T* p = nullptr;
@@ -146,7 +169,10 @@ struct Ctx {
     int single_tester = 1;
     int flip_items = 1;
     int ins_items = 1;
-    explicit Ctx(hipStream_t s);
+    // fixed: use these pinned pages (kept by the caller) instead of a slot
+    // from the free list
+    explicit Ctx(hipStream_t s, const HostSlot* fixed = nullptr);
+    bool fixed_slot = false;
     ~Ctx();
     Ctx(const Ctx&) = delete;
     Ctx& operator=(const Ctx&) = delete;
@@ -314,5 +340,7 @@ struct SolveOut {
 };
 void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, int3* tris_out, int3* adj_out,
                 SolveOut& out, int* cflag_out = nullptr);
+// set-up of gcdt_solve_noalloc on the current device (from gcdt_pool_init)
+void noalloc_setup();
 
 }  // namespace gcdt

@@ -106,6 +106,7 @@ SIGNATURES = {
     "gcdt_add_constraints": ([vp, sz, vp, vp, vp, vp, vp, vp, vp, pvp, i32, i32, i32, pint, pdbl], i32),
     "gcdt_solve": ([vp, i32, vp, i32, vp, vp, pint, ctypes.POINTER(Stats), vp], i32),
     "gcdt_solve_flags": ([vp, i32, vp, i32, vp, vp, vp, pint, ctypes.POINTER(Stats), vp], i32),
+    "gcdt_solve_noalloc": ([vp, i32, vp, i32, vp, vp, pint, ctypes.POINTER(Stats), vp], i32),
     "gcdt_carve": ([vp, vp, vp, i32, i32, vp, pint, vp], i32),
     "gcdt_solve_batch": ([vp, ctypes.POINTER(ctypes.c_int64), i32, vp, ctypes.POINTER(ctypes.c_int64), vp,
                           ctypes.POINTER(BatchStats), vp, vp], i32),

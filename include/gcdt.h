@@ -203,6 +203,26 @@ int gcdt_solve(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcd
 int gcdt_solve_flags(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcdt_int3* tris_out,
                      gcdt_int3* adj_out, int* cflag_out, int* num_tris, gcdt_stats* stats, void* stream);
 
+/* gcdt_solve that allocates nothing: no counterpart in the reference.  It is
+ * for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  Every buffer
+ * of the solve comes from a static device array of GCDT_NOALLOC_SCRATCH_BYTES
+ * per device (`scratch`, gcdt_amd/csrc/noalloc.hip), held from library load
+ * until the process ends; gcdt_pool_init() must have run once on the device
+ * first (it looks up the array's device address and reserves pinned counter
+ * pages), else GCDT_ERR_INVALID.  Arguments, outputs and status codes are
+ * those of gcdt_solve.  The array holds about 40 MB plus 260 bytes per point,
+ * plus about 48 bytes per triangle that a constraint segment crosses: inputs
+ * above GCDT_NOALLOC_MAX_POINTS points or segments fail with
+ * GCDT_ERR_SCRATCH at once, and an input whose constraints need more than the
+ * array fails with GCDT_ERR_SCRATCH naming the bytes it needed.  All calls on
+ * a device share its array: they are serialised, each returning after its
+ * stream has drained.  stats->ms_* come from the host clock. */
+#define GCDT_NOALLOC_SCRATCH_BYTES (256ull << 20)
+#define GCDT_NOALLOC_MAX_POINTS 750000
+int gcdt_solve_noalloc(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcdt_int3* tris_out,
+                       gcdt_int3* adj_out, int* num_tris, gcdt_stats* stats, void* stream);
+
 /* Even-odd carving (no counterpart in the reference, which keeps every
  * triangle of the bounding square): inside[t] = 1 for the triangles enclosed
  * by an odd number of constraint boundaries -- polygon interiors minus their

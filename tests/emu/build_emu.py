@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "gcdt_amd", "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libgcdt_emu.so")
-SOURCES = ["mesh.hip", "cons.hip", "carve.hip", "io.hip", "capi.cpp"]
+SOURCES = ["mesh.hip", "cons.hip", "carve.hip", "io.hip", "noalloc.hip", "capi.cpp"]
 HEADERS = ["gcdt_common.hpp", "predicates.hpp", "mesh_bodies.hpp", "cons_bodies.hpp", "mesh.hpp"]
 CXX = os.environ.get("CXX_EMU", "g++")
 FLAGS = ["-std=c++17", "-O1", "-g", "-fPIC", "-Wno-unknown-pragmas", "-Wno-attributes", "-fno-strict-aliasing",

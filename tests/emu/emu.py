@@ -103,7 +103,7 @@ class Buf:
 
 
 def solve(g, cons, entry="gcdt_solve"):
-    """entry: the name of a solve entry point with gcdt_solve's arguments."""
+    """entry: gcdt_solve or gcdt_solve_noalloc (same arguments)."""
     g = np.ascontiguousarray(g, np.int32).reshape(-1, 2)
     c = np.ascontiguousarray(cons, np.int32).reshape(-1, 2)
     n = len(g)
@@ -163,6 +163,32 @@ def run_case(case: dict) -> dict:
         set_knob(k, v)
     res = {"valid": True, "unique_mismatch": 0}
     kind = case["kind"]
+    if kind == "noalloc":
+        # the allocation-free solve against gcdt_solve on a sequence of inputs
+        # (each call runs over the previous call's leftovers in the array)
+        if case.get("init", True):
+            check(load().gcdt_pool_init(), "gcdt_pool_init")
+        res["status"] = []
+        for spec in case["inputs"]:
+            if spec[0] == "uniform":
+                p, c = gen.uniform_cdt(n=spec[1], c=spec[2], seed_pts=spec[3], seed_cons=spec[3] + 1)
+            elif spec[0] == "star":
+                p, c = gen.star_instance(gen.Stream(spec[2]), spec[1])
+            else:
+                p, c = gen.lattice_cdt(k=spec[1], c=spec[2], max_len=spec[3])
+            g = gen.to_grid(p)
+            try:
+                t1, a1, _ = solve(g, c, entry="gcdt_solve_noalloc")
+            except L.GcdtError as e:
+                res["status"].append(e.status)
+                continue
+            res["status"].append(0)
+            t0, _, _ = solve(g, c)
+            v = O.validate(g, t1, c)
+            res["valid"] &= bool(v["valid"]) and bool(O.validate(g, t0, c)["valid"])
+            if v["valid"] and v["unique"]:
+                res["unique_mismatch"] += int(not np.array_equal(O.canonical(t0), O.canonical(t1)))
+        return res
     if kind == "batch":
         inst = _instances(case)
         tris, st = solve_batch(inst)

@@ -97,3 +97,22 @@ def test_detector_catches_a_wrong_size_release():
     r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
     assert r.returncode == 91, (r.returncode, r.stderr)
 
+
+def test_noalloc_solve_equals_solve_on_leftovers():
+    """gcdt_solve_noalloc (every buffer carved from the static scratch array)
+    against gcdt_solve on three inputs in a row, the later ones smaller, so
+    they run over the earlier calls' contents of the array."""
+    out = _run({"kind": "noalloc", "inputs": [["uniform", 30000, 300, 51], ["star", 8000, 52],
+                                              ["lattice", 40, 20, 12]]},
+               seed=4, conc=8, yield_atomics=1)
+    assert out["status"] == [0, 0, 0], out
+
+
+def test_noalloc_solve_refuses_without_setup_and_above_its_maximum():
+    """Without gcdt_pool_init on the device there is no scratch address: the
+    call fails with GCDT_ERR_INVALID instead of touching the array; and an
+    input above GCDT_NOALLOC_MAX_POINTS fails with GCDT_ERR_SCRATCH."""
+    rc, out, err = emu.run_in_child({"kind": "noalloc", "init": False, "inputs": [["uniform", 1000, 10, 53]]})
+    assert rc == 0 and out["status"] == [2], (rc, out, err)
+    rc, out, err = emu.run_in_child({"kind": "noalloc", "inputs": [["uniform", 800000, 0, 54]]})
+    assert rc == 0 and out["status"] == [6], (rc, out, err)
