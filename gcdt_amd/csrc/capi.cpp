@@ -290,7 +290,7 @@ void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, i
     Timer t1(c.st);
     DBuf<int> sid(1 << 16);
     int S = sample_points_morton(c, cell_end.p, sid.p);
-    cell_end.release();
+    // cell_end stays for the constraints' split pass
     DBuf<int> loc((size_t)N + 4), pend((size_t)(N > 1 ? N : 1)), pend2((size_t)(N > 1 ? N : 1));
     DBuf<unsigned long long> pkey((size_t)N + 4);
     long long capA = 4LL * S + 8;
@@ -380,7 +380,7 @@ void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, i
         }
         fill_i32(c, touched.p, 0, T);
         Mesh m{pts, tris_out, adj_out, nullptr, cflag};
-        ConsResult cr = insert_constraints(c, m, T, N + 4, N, cons, C, touched.p, nullptr);
+        ConsResult cr = insert_constraints(c, m, T, N + 4, N, cons, C, touched.p, cell_end.p);
         st.constraint_rounds = cr.rounds;
         st.segments = cr.pieces;
         st.splits = cr.pieces - C;

@@ -164,6 +164,331 @@ __global__ void k_cav_begin(int* cnt)
     cnt[C_TOTAL] = 0;
 }
 
+// ---------------------------------------------------------------- split by cells
+//
+// The split pass above walks each segment on one thread, vertex by vertex: a
+// segment that runs through many collinear vertices (the lattice's diagonals:
+// about 190 each) is a long chain of dependent loads on a single thread.  When
+// the points are in Morton order of their 1024^2 cells (the single solve), the
+// vertices on a segment are found instead by testing the points of every cell
+// the segment touches, one (segment, cell) pair per thread; each segment's
+// vertices are then sorted along it in LDS, and each piece between two
+// consecutive ones is checked for being an existing edge on a thread of its
+// own.  The pieces are the same as the walk's: in order along the segment,
+// existing edges marked constrained and stored with a negative first vertex.
+// Segments with more than SEG_SORT_MAX vertices on them keep the walk.
+
+constexpr int SEG_SORT_MAX = 1024;
+
+__host__ __device__ inline long long floor_div_ll(long long a, long long b)  // b > 0
+{
+    long long q = a / b;
+    return (a % b != 0 && a < 0) ? q - 1 : q;
+}
+
+// Cells of column cx that the closed segment AB (A.x <= B.x) can touch: rows
+// [lo, hi] of the y-range the segment spans inside the column (a superset).
+__host__ __device__ inline void seg_column(int2 A, int2 B, int cx, int& lo, int& hi)
+{
+    if (A.x == B.x) {
+        lo = min(A.y, B.y) >> 20;
+        hi = max(A.y, B.y) >> 20;
+        return;
+    }
+    long long dx = (long long)B.x - A.x, dy = (long long)B.y - A.y;
+    long long xl = max((long long)A.x, (long long)cx << 20), xr = min((long long)B.x, ((long long)(cx + 1) << 20) - 1);
+    long long nl = (long long)A.y * dx + (xl - A.x) * dy, nr = (long long)A.y * dx + (xr - A.x) * dy;
+    long long ymin = floor_div_ll(min(nl, nr), dx), ymax = -floor_div_ll(-max(nl, nr), dx);
+    lo = (int)max(0LL, min(1023LL, ymin >> 20));
+    hi = (int)max(0LL, min(1023LL, ymax >> 20));
+}
+
+__device__ inline bool seg_valid(int2 e, int nmax) { return e.x >= 0 && e.y >= 0 && e.x < nmax && e.y < nmax && e.x != e.y; }
+
+// endpoints ordered by x (the column walk's direction)
+__device__ inline void seg_ends(const int2* pts, int2 e, int2& A, int2& B)
+{
+    A = pts[e.x];
+    B = pts[e.y];
+    if (A.x > B.x) {
+        int2 t = A;
+        A = B;
+        B = t;
+    }
+}
+
+__device__ inline unsigned cell_morton(unsigned x, unsigned y)
+{
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    y = (y | (y << 8)) & 0x00FF00FFu;
+    y = (y | (y << 4)) & 0x0F0F0F0Fu;
+    y = (y | (y << 2)) & 0x33333333u;
+    y = (y | (y << 1)) & 0x55555555u;
+    return x | (y << 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_seg_ncells(const int2* pts, const int2* cons, int C, int nmax, int* cnt,
+                                                       int* err)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < C; i += (long long)gridDim.x * blockDim.x) {
+        int2 e = cons[i];
+        if (!seg_valid(e, nmax)) {
+            atomicOr(err, MERR_INVALID);
+            cnt[i] = 0;
+            continue;
+        }
+        int2 A, B;
+        seg_ends(pts, e, A, B);
+        int k = 0;
+        for (int cx = A.x >> 20; cx <= (B.x >> 20); cx++) {
+            int lo, hi;
+            seg_column(A, B, cx, lo, hi);
+            k += hi - lo + 1;
+        }
+        cnt[i] = k;
+    }
+}
+
+// one (segment, Morton cell) pair per entry
+__global__ __launch_bounds__(kBlock) void k_seg_cells(const int2* pts, const int2* cons, int C, int nmax, const int* off,
+                                                      int* pair_seg, int* pair_cell)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < C; i += (long long)gridDim.x * blockDim.x) {
+        int2 e = cons[i];
+        if (!seg_valid(e, nmax)) continue;
+        int2 A, B;
+        seg_ends(pts, e, A, B);
+        int o = off[i];
+        for (int cx = A.x >> 20; cx <= (B.x >> 20); cx++) {
+            int lo, hi;
+            seg_column(A, B, cx, lo, hi);
+            for (int cy = lo; cy <= hi; cy++) {
+                pair_seg[o] = (int)i;
+                pair_cell[o] = (int)cell_morton((unsigned)cx, (unsigned)cy);
+                o++;
+            }
+        }
+    }
+}
+
+// The points of each pair's cell that lie strictly inside its segment: counted
+// per segment (pos == nullptr), or written to hits[hoff[seg] + ...].
+__global__ __launch_bounds__(kBlock) void k_seg_hits(const int2* pts, const int* cell_end, const int2* cons,
+                                                     const int* pair_seg, const int* pair_cell, long long npairs,
+                                                     int* hcnt, const int* hoff, int* hits)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < npairs; i += (long long)gridDim.x * blockDim.x) {
+        int sg = pair_seg[i], cell = pair_cell[i];
+        int2 e = cons[sg];
+        int2 A = pts[e.x], B = pts[e.y];
+        int p0 = cell ? cell_end[cell - 1] : 0, p1 = cell_end[cell];
+        for (int p = p0; p < p1; p++) {
+            if (p == e.x || p == e.y) continue;
+            int2 P = pts[p];
+            if (orient2d(A, B, P) != 0 || !strictly_between(A, B, P)) continue;
+            int k = atomicAdd(&hcnt[sg], 1);
+            if (hits) hits[hoff[sg] + k] = p;
+        }
+    }
+}
+
+// pieces per segment: h + 1 for a sorted segment; a segment with more than
+// SEG_SORT_MAX vertices on it goes to the walk list (its count comes later)
+__global__ __launch_bounds__(kBlock) void k_seg_npieces(const int2* cons, int C, int nmax, const int* hcnt, int* np,
+                                                        int* sort_list, int* nsort, int* walk_list, int* nwalk)
+{
+    WAVE_LOOP(i, C)
+    {
+        int h = 0;
+        bool valid = false;
+        if (i < C) {
+            valid = seg_valid(cons[i], nmax);
+            h = hcnt[i];
+            np[i] = valid ? (h <= SEG_SORT_MAX ? h + 1 : 0) : 0;
+        }
+        bool srt = valid && h >= 2 && h <= SEG_SORT_MAX, wlk = valid && h > SEG_SORT_MAX;
+        int a = wave_alloc(srt ? 1 : 0, nsort);
+        if (srt) sort_list[a] = (int)i;
+        int b = wave_alloc(wlk ? 1 : 0, nwalk);
+        if (wlk) walk_list[b] = (int)i;
+    }
+}
+
+// one workgroup per listed segment: its vertices sorted along it (bitonic
+// sort in LDS of (position along the segment, vertex) pairs)
+__global__ __launch_bounds__(kBlock) void k_seg_sort(const int2* pts, const int2* cons, const int* list, const int* nlist,
+                                                     const int* hcnt, const int* hoff, int* hits)
+{
+    __shared__ long long s_key[SEG_SORT_MAX];
+    __shared__ int s_val[SEG_SORT_MAX];
+    int n = *nlist;
+    for (int w = blockIdx.x; w < n; w += gridDim.x) {
+        int sg = list[w];
+        int h = hcnt[sg], o = hoff[sg];
+        int2 e = cons[sg];
+        int2 A = pts[e.x], B = pts[e.y];
+        long long dx = (long long)B.x - A.x, dy = (long long)B.y - A.y;
+        int m = 1;
+        while (m < h) m <<= 1;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            if (k < h) {
+                int p = hits[o + k];
+                int2 P = pts[p];
+                s_key[k] = ((long long)P.x - A.x) * dx + ((long long)P.y - A.y) * dy;
+                s_val[k] = p;
+            } else {
+                s_key[k] = LLONG_MAX;
+                s_val[k] = -1;
+            }
+        }
+        __syncthreads();
+        for (int size = 2; size <= m; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int k = threadIdx.x; k < m; k += blockDim.x) {
+                    int j = k ^ stride;
+                    if (j > k) {
+                        bool up = (k & size) == 0;
+                        long long a = s_key[k], b = s_key[j];
+                        if ((a > b) == up) {
+                            s_key[k] = b;
+                            s_key[j] = a;
+                            int t = s_val[k];
+                            s_val[k] = s_val[j];
+                            s_val[j] = t;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int k = threadIdx.x; k < h; k += blockDim.x) hits[o + k] = s_val[k];
+        __syncthreads();
+    }
+}
+
+__device__ inline int upper_bound_seg(const int* a, int n, long long x)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// one thread per piece of the sorted segments: (previous vertex, next vertex)
+// along the segment; an existing edge is marked constrained and stored with
+// a negative first vertex
+__global__ __launch_bounds__(kBlock) void k_seg_pieces(Cons c, const int2* cons, int C, const int* poff,
+                                                       const int* hcnt, const int* hoff, const int* hits,
+                                                       long long total, int2* pieces)
+{
+    const Mesh& m = c.m;
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (long long)gridDim.x * blockDim.x) {
+        int sg = upper_bound_seg(poff, C + 1, j) - 1;
+        int h = hcnt[sg];
+        if (h > SEG_SORT_MAX) continue;  // the walk writes this segment's pieces
+        int k = (int)(j - poff[sg]);
+        int2 e = cons[sg];
+        // the q-th vertex from e.x (hits are sorted from e.x towards e.y)
+        auto vtx = [&](int q) { return q == 0 ? e.x : (q == h + 1 ? e.y : hits[hoff[sg] + q - 1]); };
+        int u = vtx(k), v = vtx(k + 1);
+        int t, s;
+        int r = find_wedge(c, u, v, t, s);
+        if (r < 0) {
+            at_or(c.err, MERR_WALK);
+            pieces[j] = make_int2(u, v);
+            continue;
+        }
+        if (r == 1 && get3(m.tri[t], nx3(s)) == v) {
+            mark_pair(c, t, pv3(s));
+            pieces[j] = make_int2(-(u + 1), v);
+        } else {
+            pieces[j] = make_int2(u, v);
+        }
+    }
+}
+
+// the walk's pieces for the listed (long) segments: counted into np, then
+// written at their offsets
+__global__ __launch_bounds__(kBlock) void k_seg_walk(Cons c, const int2* cons, const int* list, const int* nlist,
+                                                     int* np, const int* poff, int2* pieces)
+{
+    int n = *nlist;
+    for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (long long)gridDim.x * blockDim.x) {
+        int sg = list[w];
+        int2 e = cons[sg];
+        if (poff) split_walk(c, e.x, e.y, pieces + poff[sg]);
+        else np[sg] = split_walk(c, e.x, e.y, nullptr);
+    }
+}
+
+static int split_pieces_cells(Ctx& cx, const Cons& c, const int* cell_end, const int2* cons, int C, int nmax,
+                              DBuf<int2>& pieces)
+{
+    const int2* pts = c.m.pts;
+    DBuf<int> off((size_t)C + 1);
+    k_seg_ncells<<<grid_for(C), kBlock, 0, cx.st>>>(pts, cons, C, nmax, off.p, c.err);
+    GCDT_LAUNCH_CHECK();
+    cx.check_err();
+    scan_excl_total(cx, off.p, off.p, C);
+    long long npairs = cx.get(C_TOTAL);
+    DBuf<int> pseg(npairs > 0 ? npairs : 1), pcell(npairs > 0 ? npairs : 1);
+    k_seg_cells<<<grid_for(C), kBlock, 0, cx.st>>>(pts, cons, C, nmax, off.p, pseg.p, pcell.p);
+    GCDT_LAUNCH_CHECK();
+    off.release();
+    // vertices strictly inside each segment: count, offsets, positions
+    DBuf<int> hcnt((size_t)C + 1), hoff((size_t)C + 1);
+    fill_i32(cx, hcnt.p, 0, C);
+    k_seg_hits<<<grid_for(npairs), kBlock, 0, cx.st>>>(pts, cell_end, cons, pseg.p, pcell.p, npairs, hcnt.p, nullptr,
+                                                       nullptr);
+    GCDT_LAUNCH_CHECK();
+    scan_excl_total(cx, hcnt.p, hoff.p, C);
+    long long nh = cx.get(C_TOTAL);
+    DBuf<int> hits(nh > 0 ? nh : 1);
+    fill_i32(cx, hcnt.p, 0, C);
+    k_seg_hits<<<grid_for(npairs), kBlock, 0, cx.st>>>(pts, cell_end, cons, pseg.p, pcell.p, npairs, hcnt.p, hoff.p,
+                                                       hits.p);
+    GCDT_LAUNCH_CHECK();
+    pseg.release();
+    pcell.release();
+    // pieces per segment; segments with many vertices keep the walk
+    DBuf<int> np((size_t)C + 1), slist(C > 0 ? C : 1), wlist(C > 0 ? C : 1);
+    cx.set(C_NQ, 0);
+    cx.set(C_NQ2, 0);
+    k_seg_npieces<<<grid_for(C), kBlock, 0, cx.st>>>(cons, C, nmax, hcnt.p, np.p, slist.p, cx.dcnt.p + C_NQ, wlist.p,
+                                                     cx.dcnt.p + C_NQ2);
+    GCDT_LAUNCH_CHECK();
+    k_seg_sort<<<std::min(std::max(C, 1), 65536), kBlock, 0, cx.st>>>(pts, cons, slist.p, cx.dcnt.p + C_NQ, hcnt.p,
+                                                                      hoff.p, hits.p);
+    GCDT_LAUNCH_CHECK();
+    cx.read(C_NQ, 2);
+    int nwalk = cx.hcnt[C_NQ2];
+    if (nwalk > 0) {
+        k_seg_walk<<<grid_for(nwalk), kBlock, 0, cx.st>>>(c, cons, wlist.p, cx.dcnt.p + C_NQ2, np.p, nullptr, nullptr);
+        GCDT_LAUNCH_CHECK();
+        cx.check_err();
+    }
+    scan_excl_total(cx, np.p, np.p, C);
+    int total = cx.get(C_TOTAL);
+    pieces.alloc(total > 0 ? total : 1);
+    if (total > 0) {
+        k_seg_pieces<<<grid_for(total), kBlock, 0, cx.st>>>(c, cons, C, np.p, hcnt.p, hoff.p, hits.p, total, pieces.p);
+        GCDT_LAUNCH_CHECK();
+    }
+    if (nwalk > 0) {
+        k_seg_walk<<<grid_for(nwalk), kBlock, 0, cx.st>>>(c, cons, wlist.p, cx.dcnt.p + C_NQ2, nullptr, np.p,
+                                                          pieces.p);
+        GCDT_LAUNCH_CHECK();
+    }
+    cx.check_err();
+    return total;
+}
+
 // Split pass: pieces of every constraint (existing edges marked constrained
 // and stored with a negative first vertex).  Returns the piece count.
 static int split_pieces(Ctx& cx, const Cons& c, const int2* cons, int C, int nmax, DBuf<int2>& pieces)
@@ -206,9 +531,8 @@ __global__ __launch_bounds__(kBlock) void k_touched(const int* stamp, int T, int
 }
 
 ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C,
-                              int* touched, int* ntouched)
+                              int* touched, const int* cell_end)
 {
-    (void)ntouched;
     ConsResult res;
     if (C <= 0) return res;
     DBuf<int> vt(nv);
@@ -228,7 +552,8 @@ ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, int nmax, c
     c.err = cx.dcnt.p + C_ERR;
     // 1. split into pieces
     DBuf<int2> pieces;
-    int np = split_pieces(cx, c, cons, C, nmax, pieces);
+    int np = cell_end ? split_pieces_cells(cx, c, cell_end, cons, C, nmax, pieces)
+                      : split_pieces(cx, c, cons, C, nmax, pieces);
     res.pieces = np;
     // 2. rounds over pieces that need a cavity
     DBuf<int> qa(np > 0 ? np : 1), qb(np > 0 ? np : 1), ncross(np > 0 ? np : 1);

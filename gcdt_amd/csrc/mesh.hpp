@@ -325,8 +325,10 @@ struct ConsResult {
     long long rewritten = 0;
 };
 // nmax: constraint endpoints must be vertex ids in [0, nmax)
+// cell_end (optional): the Morton cell ends of m.pts (morton_order), so the
+// split pass finds the vertices on each segment by cells instead of walking
 ConsResult insert_constraints(Ctx& c, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C, int* touched,
-                              int* ntouched);
+                              const int* cell_end);
 // mark every edge chain of the mesh that runs exactly along a constraint
 // (constraints split at collinear vertices) as constrained; returns pieces
 long long mark_constraint_chains(Ctx& c, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C);
