@@ -101,14 +101,36 @@ def _time(run, repeats):
     return ts[len(ts) // 2]
 
 
+def _confirm(run, start: dict, choice: dict, repeats: int, min_gain: float):
+    """The descent's end point against its starting point, timed in
+    alternation (A B A B ...) so that drift and bimodal run times weigh on both
+    alike; the choice stands only if its median is at least min_gain below the
+    start's.  Returns (kept choice, its median seconds, the start's)."""
+    ta, tb = [], []
+    for _ in range(repeats):
+        for cfg, ts in ((start, ta), (choice, tb)):
+            apply(cfg)
+            t0 = time.perf_counter()
+            run()
+            ts.append(time.perf_counter() - t0)
+    ta.sort()
+    tb.sort()
+    ma, mb = ta[len(ta) // 2], tb[len(tb) // 2]
+    return (choice, mb, ma) if mb < (1.0 - min_gain) * ma else (start, ma, ma)
+
+
 def tune(kind: str, run, candidates=None, repeats: int = 5, key: str | None = None, path: str | None = None,
-         min_gain: float = 0.02):
+         min_gain: float = 0.02, confirm_repeats: int = 9):
     """Time `run()` (one complete solve, synchronised) over the candidates and
     apply the fastest; a knob leaves its current value only for a candidate
-    at least `min_gain` faster (run-to-run noise is ~1-2%).  Returns
-    {"choice": {...}, "ms": best, "trials": [...]}."""
+    at least `min_gain` faster (run-to-run noise is ~1-2%).  The end point
+    is then timed against the starting values in alternation and kept only if
+    it is still at least `min_gain` faster (single trials of a solve whose
+    flip tail is bimodal can look faster by chance).  Returns
+    {"choice": {...}, "ms": best, "trials": [...], "confirmed": bool}."""
     candidates = candidates or CANDIDATES
     choice = {k: get(k) for k in candidates}
+    start = dict(choice)
     trials = []
     run()  # warm code objects and the pool at the current setting
     for name, values in candidates.items():
@@ -130,9 +152,16 @@ def tune(kind: str, run, candidates=None, repeats: int = 5, key: str | None = No
         best_v = min(times, key=times.get)
         if times[best_v] < (1.0 - min_gain) * times[cur]:
             choice[name] = best_v
+    confirmed = True
+    if choice != start and confirm_repeats > 0:
+        kept, t_kept, t_start = _confirm(run, start, choice, confirm_repeats, min_gain)
+        confirmed = kept == choice
+        trials.append({"confirm": True, "start_ms": round(t_start * 1000.0, 3), "choice_ms": round(t_kept * 1000.0, 3)
+                       if confirmed else None})
+        choice = dict(kept)
     apply(choice)
-    best_ms = min(t["ms"] for t in trials if all(t[k] == choice[k] for k in choice))
-    res = {"choice": choice, "ms": best_ms, "trials": trials}
+    best_ms = min((t["ms"] for t in trials if "ms" in t and all(t[k] == choice[k] for k in choice)), default=None)
+    res = {"choice": choice, "ms": best_ms, "trials": trials, "confirmed": confirmed}
     path = path or cache_path()
     data = _load(path)
     data.setdefault(key or device_key(), {})[kind] = {"choice": choice, "ms": best_ms}

@@ -74,3 +74,20 @@ def test_only_gpu_tested_values_are_accepted():
     finally:
         for k, v in saved.items():
             set_tunable(k, v)
+
+
+def test_a_gain_that_does_not_hold_is_not_kept(tmp_path, fake_knobs):
+    """A value that timed faster during the descent but not when timed again
+    against the starting values (in alternation) is dropped: the tuner keeps
+    the start."""
+    calls = [0]
+    cand = {"tail_threshold": [1024, 4096, 8192]}
+
+    def run():
+        calls[0] += 1
+        lucky = calls[0] <= 16 and fake_knobs["tail_threshold"] == 8192  # only while descending
+        time.sleep(0.001 if lucky else 0.004)
+
+    res = tune.tune("unit", run, candidates=cand, repeats=3, key="cpu-test", path=str(tmp_path / "t.json"))
+    assert res["choice"] == {"tail_threshold": 4096} and not res["confirmed"]
+    assert fake_knobs["tail_threshold"] == 4096
