@@ -1,10 +1,16 @@
 """Seeded synthetic CDT workloads (points + constraint segments).
 
-All randomness comes from SplitMix64 streams so that every workload is
-reproducible bit for bit on any machine; no STL/numpy distribution is used for
-integer decisions (the reference's generators, kernel.cu:43-170, rely on
+All randomness comes from SplitMix64 streams; no STL/numpy distribution is
+used (the reference's generators, kernel.cu:43-170, rely on
 implementation-defined std::default_random_engine streams and cannot be
-reproduced).  Coordinates are integers in [0, 2^20); :func:`to_grid` maps them
+reproduced).  The point sets of the uniform, clustered and lattice workloads
+and every polygon built from circles are computed in integer arithmetic only
+(Irwin-Hall Gaussian offsets, fixed-point cos/sin), so they are the same on
+every machine.  The star instances' vertex angles and the log-uniform sizes and
+Poisson arrival times of the batched and serving workloads use double
+arithmetic and the C library's cos/sin/log/exp, rounded to integers; a last-bit
+difference between C libraries could move a value that lies within one ulp of
+a rounding boundary.  Coordinates are integers in [0, 2^20); :func:`to_grid` maps them
 exactly into the reference's 2^30 point grid (kernel.cu:171-188 maps doubles
 into the same grid) by x' = (x << 10) + 512, a positive scale plus a
 translation, so every orientation / in-circle sign is preserved and every
@@ -127,52 +133,108 @@ def uniform_cdt(n: int = 100_000, c: int = 1_000, seed_pts: int = 1, seed_cons: 
     return _finish([pts], segs)
 
 
-def _circle_segments(s: Stream, count: int, k: int, rmin: float, rmax: float, taken=None):
-    """count disjoint k-gons (vertices rounded to the grid) with pairwise
-    disjoint bounding boxes; returns list of ((x1,y1),(x2,y2)) segments."""
+# Fixed-point trigonometry in Python integers (exact and the same on every
+# machine, unlike a C library's cos/sin): values scaled by 2^TRIG_BITS.
+TRIG_BITS = 40
+_WORK = 96  # working precision of the series
+
+
+def _atan_inv(x: int) -> int:
+    """atan(1/x) * 2^_WORK by its alternating series."""
+    one = 1 << _WORK
+    term = one // x
+    total, k, x2 = 0, 0, x * x
+    while term:
+        total += term // (2 * k + 1) if k % 2 == 0 else -(term // (2 * k + 1))
+        term //= x2
+        k += 1
+    return total
+
+
+_PI = 16 * _atan_inv(5) - 4 * _atan_inv(239)  # Machin's formula, pi * 2^_WORK
+
+
+def cos_sin_fixed(k: int, m: int):
+    """(cos, sin) of 2*pi*k/m, each round(v * 2^TRIG_BITS), from Taylor series
+    in integer arithmetic."""
+    one = 1 << _WORK
+    k %= m
+    if 2 * k > m:  # angle in (-pi, pi]
+        k -= m
+    a = (2 * _PI * k) // m
+    c, s_, term, n = 0, 0, one, 0
+    while term:
+        if n % 2 == 0:
+            c += term if n % 4 == 0 else -term
+        else:
+            s_ += term if n % 4 == 1 else -term
+        n += 1
+        q = abs(term * a) // (n << _WORK)  # truncated towards zero, so the series ends
+        term = q if term * a >= 0 else -q
+    sh = _WORK - TRIG_BITS
+    half = 1 << (sh - 1)
+    return (c + half) >> sh, (s_ + half) >> sh
+
+
+def _circle_segments(s: Stream, count: int, k: int, rmin: int, rmax: int, taken=None):
+    """count disjoint k-gons with pairwise disjoint bounding boxes, all in
+    integer arithmetic: integer centre and radius, vertex j at
+    centre + round(r * (cos, sin)(2*pi*j/k)) from cos_sin_fixed.  Returns a
+    list of ((x1,y1),(x2,y2)) segments."""
     boxes = [] if taken is None else taken
     segs = []
     tries = 0
+    cs = np.array([cos_sin_fixed(j, k) for j in range(k)], dtype=np.int64)
+    half = 1 << (TRIG_BITS - 1)
     while len(boxes) < count and tries < 200 * count + 1000:
         tries += 1
-        r = rmin + (rmax - rmin) * float(s.unit(1)[0])
-        cx, cy = (s.unit(2) * (COORD_MAX - 2 * r - 4) + r + 2).tolist()
+        r = int(rmin + s.below(1, rmax - rmin + 1)[0])
+        cx, cy = (s.below(2, COORD_MAX - 2 * r - 4) + r + 2).tolist()
         box = (cx - r - 2, cy - r - 2, cx + r + 2, cy + r + 2)
         if any(not (box[2] < b[0] or b[2] < box[0] or box[3] < b[1] or b[3] < box[1]) for b in boxes):
             continue
         boxes.append(box)
-        ang = 2.0 * math.pi * np.arange(k) / k
-        vx = np.rint(cx + r * np.cos(ang)).astype(np.int64)
-        vy = np.rint(cy + r * np.sin(ang)).astype(np.int64)
+        vx = cx + ((r * cs[:, 0] + half) >> TRIG_BITS)
+        vy = cy + ((r * cs[:, 1] + half) >> TRIG_BITS)
         for i in range(k):
             j = (i + 1) % k
             segs.append(((int(vx[i]), int(vy[i])), (int(vx[j]), int(vy[j]))))
     return segs
 
 
+def gaussian_offsets(s: Stream, n: int, log2_sigma: int = 12) -> np.ndarray:
+    """n integer offsets, approximately normal with standard deviation
+    2^log2_sigma, in integer arithmetic only: the sum of 12 uniform 16-bit
+    integers (variance 12 * (2^32 - 1) / 12, about 2^32) centred and scaled
+    by 2^(log2_sigma - 16) with rounding (the Irwin-Hall construction; its
+    tails end at 6 sigma)."""
+    acc = np.zeros(n, dtype=np.int64)
+    for _ in range(12):
+        acc += s.bits(n, 16)
+    g = acc - 12 * ((1 << 16) - 1) // 2
+    sh = 16 - log2_sigma
+    return (g + (1 << (sh - 1))) >> sh
+
+
 def clustered_cdt(n: int = 10_000_000, circles: int = 1_000, k: int = 100, seed_pts: int = 3,
-                  seed_cons: int = 4, sigma: float = 4096.0, centres: int = 64):
-    """Plain CDT, large: 80% uniform / 20% Gaussian-cluster points plus
-    `circles` disjoint k-gons (radius >= 2^9) as constraints."""
+                  seed_cons: int = 4, log2_sigma: int = 12, centres: int = 64):
+    """Plain CDT, large: 80% uniform / 20% Gaussian-cluster points (sigma
+    2^12 around 64 uniform centres, clamped to the coordinate range) plus
+    `circles` disjoint k-gons (radius >= 2^9) as constraints.  Integer
+    arithmetic throughout."""
     s = Stream(seed_pts)
-    cen = s.unit(2 * centres).reshape(-1, 2) * COORD_MAX
-    sel = s.unit(n)
-    clustered = sel >= 0.8
+    cen = s.bits(2 * centres, COORD_BITS).reshape(-1, 2)
+    clustered = s.below(n, 5) == 4
     ux = s.bits(n, COORD_BITS)
     uy = s.bits(n, COORD_BITS)
     cid = s.below(n, centres)
-    u1 = 1.0 - s.unit(n)  # (0,1]
-    u2 = s.unit(n)
-    rad = np.sqrt(-2.0 * np.log(u1)) * sigma
-    gx = np.rint(cen[cid, 0] + rad * np.cos(2 * math.pi * u2))
-    gy = np.rint(cen[cid, 1] + rad * np.sin(2 * math.pi * u2))
-    gx = np.clip(gx, 0, COORD_MAX - 1).astype(np.int64)
-    gy = np.clip(gy, 0, COORD_MAX - 1).astype(np.int64)
+    gx = np.clip(cen[cid, 0] + gaussian_offsets(s, n, log2_sigma), 0, COORD_MAX - 1)
+    gy = np.clip(cen[cid, 1] + gaussian_offsets(s, n, log2_sigma), 0, COORD_MAX - 1)
     x = np.where(clustered, gx, ux)
     y = np.where(clustered, gy, uy)
     s2 = Stream(seed_cons)
-    rmax = min(4096.0, COORD_MAX / (4.0 * math.sqrt(max(circles, 1))))
-    segs = _circle_segments(s2, circles, k, 512.0, max(rmax, 600.0))
+    rmax = min(4096, COORD_MAX // (4 * math.isqrt(max(circles, 1))))
+    segs = _circle_segments(s2, circles, k, 512, max(rmax, 600))
     return _finish([np.stack([x, y], axis=1)], segs)
 
 
@@ -282,7 +344,7 @@ def polygon_instance(s: Stream, n: int, frac_lo: float = 0.01, frac_hi: float = 
         k = int(min(want - len(segs), 16 + int(s.below(1, 85)[0])))
         k = max(k, 3)
         before = len(boxes)
-        segs += _circle_segments(s, before + 1, k, 64.0, 8192.0, taken=boxes)
+        segs += _circle_segments(s, before + 1, k, 64, 8192, taken=boxes)
         if len(boxes) == before:
             break
     return _finish([pts], segs)
