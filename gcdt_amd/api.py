@@ -84,6 +84,53 @@ def solve(pts_grid: np.ndarray, cons: np.ndarray | None = None, with_adj: bool =
     return (tris, st, adj) if with_adj else (tris, st)
 
 
+def hpoint_coords(h) -> tuple[int, int, int]:
+    """(X, Y, W) Python integers of a gcdt_hpoint: the point (X / W, Y / W)."""
+    x = (int(h["x_hi"]) << 64) | int(h["x_lo"])
+    y = (int(h["y_hi"]) << 64) | int(h["y_lo"])
+    return x, y, int(h["w"])
+
+
+HPOINT_DTYPE = np.dtype([("x_hi", "<i8"), ("x_lo", "<u8"), ("y_hi", "<i8"), ("y_lo", "<u8"), ("w", "<i8"),
+                         ("seg", "<i4", (4,))])
+
+
+def solve_steiner(pts_grid: np.ndarray, cons: np.ndarray, max_steiner: int | None = None):
+    """CDT with crossing constraints resolved by exact crossing vertices.
+    Returns (tris (T,3) int32, crossings [(X, Y, W)] for vertex ids n+4, n+5,
+    ..., their pieces (S,4) int32, cflag (T,) int32, stats).  max_steiner
+    defaults to 4 * len(cons) + 64."""
+    p = np.ascontiguousarray(pts_grid, dtype=np.int32).reshape(-1, 2)
+    n = len(p)
+    c = np.ascontiguousarray(cons, dtype=np.int32).reshape(-1, 2)
+    ms = 4 * len(c) + 64 if max_steiner is None else int(max_steiner)
+    cap = 2 * (n + ms) + 2
+    dp = DeviceBuffer.from_array(p)
+    dc = DeviceBuffer.from_array(c) if len(c) else None
+    dt, da, df = DeviceBuffer(12 * cap), DeviceBuffer(12 * cap), DeviceBuffer(4 * cap)
+    dh = DeviceBuffer(HPOINT_DTYPE.itemsize * max(ms, 1))
+    try:
+        nt, ns = ctypes.c_int(0), ctypes.c_int(0)
+        st = Stats()
+        check(lib().gcdt_solve_steiner(dp.ptr, n, dc.ptr if dc is not None else None, len(c), ms, dt.ptr, da.ptr,
+                                       df.ptr, dh.ptr, ctypes.byref(nt), ctypes.byref(ns), ctypes.byref(st), None),
+              "gcdt_solve_steiner")
+        tris = dt.to_array(np.int32, (nt.value, 3))
+        cflag = df.to_array(np.int32, (nt.value,))
+        raw = np.zeros(ns.value, HPOINT_DTYPE)
+        if ns.value:
+            lib().gcdt_memcpy_d2h(raw.ctypes.data, dh.ptr, raw.nbytes)
+        pts = [hpoint_coords(h) for h in raw]
+        segs = raw["seg"].astype(np.int32).reshape(-1, 4)
+    finally:
+        for b in (dp, dc, dt, da, df, dh):
+            if b is not None:
+                b.free()
+    d = st.as_dict()
+    d["steiner"] = ns.value
+    return tris, pts, segs, cflag, d
+
+
 def solve_carved(pts_grid: np.ndarray, cons: np.ndarray):
     """CDT plus even-odd carving.  Returns (all triangles (2n+2,3), constraint
     bits (2n+2,), inside mask (2n+2,) bool, stats)."""

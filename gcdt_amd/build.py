@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgcdt.so")
-SOURCES = ["mesh.hip", "cons.hip", "carve.hip", "io.hip", "noalloc.hip", "capi.cpp", "memcache.cpp"]
+SOURCES = ["mesh.hip", "cons.hip", "carve.hip", "io.hip", "noalloc.hip", "steiner.hip", "capi.cpp", "memcache.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GCDT_ARCH", "gfx950")
 

@@ -271,7 +271,7 @@ namespace gcdt {
 
 // Full pipeline on device arrays; pts holds N+4 entries (corners written here).
 void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, int3* tris_out, int3* adj_out,
-                SolveOut& out, int* cflag_out)
+                SolveOut& out, int* cflag_out, const SteinerArgs* steiner)
 {
     gcdt_stats& st = out.st;
     Timer total(c.st);
@@ -373,14 +373,18 @@ void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, i
     if (C > 0) {
         DBuf<int> cflag_buf, touched(T);
         int* cflag = cflag_out;
+        // the crossing pass adds two triangles per crossing vertex
+        const long long tcap = steiner ? 2LL * ((long long)N + steiner->scap) + 2 : T;
         if (!cflag) {
-            cflag_buf.alloc(T);
+            cflag_buf.alloc((size_t)tcap);
             cflag = cflag_buf.p;
             fill_i32(c, cflag, 0, T);
         }
         fill_i32(c, touched.p, 0, T);
         Mesh m{pts, tris_out, adj_out, nullptr, cflag};
-        ConsResult cr = insert_constraints(c, m, T, N + 4, N, cons, C, touched.p, cell_end.p);
+        ConsDefer defer;
+        ConsResult cr = insert_constraints(c, m, T, N + 4, N, cons, C, touched.p, cell_end.p,
+                                           steiner ? &defer : nullptr);
         st.constraint_rounds = cr.rounds;
         st.segments = cr.pieces;
         st.splits = cr.pieces - C;
@@ -391,6 +395,11 @@ void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, i
             int f3 = 0;
             st.flips += flip_rounds(c, m, T, tl.p, nt, &f3);
             st.flip_rounds += f3;
+        }
+        if (steiner) {
+            tl.release();
+            touched.release();
+            T = steiner_pass(c, m, T, (int)tcap, N + 4, defer, steiner->scap, steiner->out, perm.p, N, &out.S);
         }
     }
     unmap_tris(c, tris_out, T, perm.p, N);
@@ -436,6 +445,35 @@ extern "C" int gcdt_solve_flags(const gcdt_int2* points, int N, const gcdt_int2*
     SolveOut out;
     solve_impl(c, (const int2*)points, N, (const int2*)cons, C, (int3*)tris_out, (int3*)adj_out, out, cflag_out);
     if (num_tris) *num_tris = out.T;
+    if (stats) *stats = out.st;
+    return GCDT_OK;
+    API_END
+}
+
+extern "C" int gcdt_solve_steiner(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, int max_steiner,
+                                  gcdt_int3* tris_out, gcdt_int3* adj_out, int* cflag_out, gcdt_hpoint* steiner_out,
+                                  int* num_tris, int* num_steiner, gcdt_stats* stats, void* stream)
+{
+    API_BEGIN
+    if (N < 0 || C < 0 || max_steiner < 0 || (N > 0 && !points) || (C > 0 && !cons) || !tris_out ||
+        (max_steiner > 0 && !steiner_out))
+        return fail(GCDT_ERR_INVALID, "bad arguments");
+    if (N > (1 << 28) || (long long)N + max_steiner > (1 << 28)) return fail(GCDT_ERR_INVALID, "too many points");
+    Ctx c((hipStream_t)stream);
+    const long long tcap = 2LL * ((long long)N + max_steiner) + 2;
+    DBuf<int3> adj_tmp;
+    int3* adj = (int3*)adj_out;
+    if (!adj) {
+        adj_tmp.alloc((size_t)tcap);
+        adj = adj_tmp.p;
+    }
+    SteinerArgs sa;
+    sa.scap = max_steiner;
+    sa.out = steiner_out;
+    SolveOut out;
+    solve_impl(c, (const int2*)points, N, (const int2*)cons, C, (int3*)tris_out, adj, out, cflag_out, &sa);
+    if (num_tris) *num_tris = out.T;
+    if (num_steiner) *num_steiner = out.S;
     if (stats) *stats = out.st;
     return GCDT_OK;
     API_END

@@ -86,10 +86,17 @@ __global__ __launch_bounds__(kBlock) void k_cav_claim(Cons c, const int2* pieces
             int pi = q[i];
             int2 pc = pieces[pi];
             unsigned long long key = cav_key(epoch, pi);
-            k = cav_walk(c, pc.x, pc.y, true, nullptr, [&](int t) { atomicMin(&c.owner[t], key); });
-            if (k < 0) {
-                atomicOr(c.err, MERR_WALK);
+            // steiner mode: a piece that crosses a recovered one is deferred
+            // before it claims anything (its claims would block others)
+            if (c.steiner && cav_walk(c, pc.x, pc.y, false, nullptr, [](int) {}) == -2) {
+                c.defer[pi] = 1;
                 k = 0;
+            } else {
+                k = cav_walk(c, pc.x, pc.y, true, nullptr, [&](int t) { atomicMin(&c.owner[t], key); });
+                if (k < 0) {
+                    atomicOr(c.err, MERR_WALK);
+                    k = 0;
+                }
             }
             ncross[i] = k;
         }
@@ -531,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_touched(const int* stamp, int T, int
 }
 
 ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C,
-                              int* touched, const int* cell_end)
+                              int* touched, const int* cell_end, ConsDefer* defer)
 {
     ConsResult res;
     if (C <= 0) return res;
@@ -550,11 +557,18 @@ ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, int nmax, c
     c.stamp = stamp.p;
     c.cavof = cavof.p;
     c.err = cx.dcnt.p + C_ERR;
+    c.steiner = defer != nullptr;
     // 1. split into pieces
     DBuf<int2> pieces;
     int np = cell_end ? split_pieces_cells(cx, c, cell_end, cons, C, nmax, pieces)
                       : split_pieces(cx, c, cons, C, nmax, pieces);
     res.pieces = np;
+    DBuf<int> dflag;
+    if (defer) {
+        dflag.alloc(np > 0 ? np : 1);
+        fill_i32(cx, dflag.p, 0, np);
+        c.defer = dflag.p;
+    }
     // 2. rounds over pieces that need a cavity
     DBuf<int> qa(np > 0 ? np : 1), qb(np > 0 ? np : 1), ncross(np > 0 ? np : 1);
     DBuf<Win> wins(np > 0 ? np : 1);
@@ -599,6 +613,11 @@ ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, int nmax, c
     if (touched) {
         k_touched<<<grid_for(T), kBlock, 0, cx.st>>>(stamp.p, T, touched);
         GCDT_LAUNCH_CHECK();
+    }
+    if (defer) {
+        defer->pieces = std::move(pieces);
+        defer->flag = std::move(dflag);
+        defer->np = np;
     }
     return res;
 }

@@ -11,6 +11,8 @@ struct Cons {
     int* stamp;                    // per triangle: epoch of its last rewrite
     int* cavof;                    // per triangle: winner index of that rewrite
     int* err;
+    int steiner = 0;               // crossing pieces are deferred instead of failing
+    int* defer = nullptr;          // per piece: 1 when deferred (steiner mode)
 };
 
 __host__ __device__ inline bool on_ray(int2 A, int2 U, int2 B)
@@ -108,7 +110,9 @@ __host__ __device__ inline int split_walk(const Cons& c, int a, int b, int2* out
         int next = -1;
         for (int g2 = 0; g2 < (1 << 26); g2++) {
             int n, q;
-            if (cross_edge(m, tt, rr, ll, n, q)) {
+            // a constrained edge crossed: an error unless crossings are
+            // resolved later (the split only looks for vertices on the segment)
+            if (cross_edge(m, tt, rr, ll, n, q) && !c.steiner) {
                 at_or(c.err, MERR_CROSSING);
                 return np;
             }
@@ -147,7 +151,8 @@ __host__ __device__ inline unsigned long long cav_key(unsigned epoch, int piece)
 
 // Walk piece (a,b) from its wedge at a; calls f(tri) for each crossed
 // triangle.  Returns the crossed count, 0 if (a,b) already is an edge (marked
-// here when mark is set), -1 on error.
+// here when mark is set), -1 on error, -2 when it meets a constrained edge
+// and crossings are deferred (c.steiner).
 template <class F>
 __host__ __device__ inline int cav_walk(const Cons& c, int a, int b, bool mark, int* t0out, F&& f)
 {
@@ -169,6 +174,7 @@ __host__ __device__ inline int cav_walk(const Cons& c, int a, int b, bool mark, 
     for (int g = 0; g < (1 << 26); g++) {
         int n, q;
         if (cross_edge(m, t, rr, ll, n, q)) {
+            if (c.steiner) return -2;
             at_or(c.err, MERR_CROSSING);
             return -1;
         }

@@ -122,6 +122,8 @@ enum : int {
     C_NPROP = 21,     // proposals of the current flip round
     C_NW = 22,        // insertion: triangles claimed for the current round
     C_NW2 = 23,       // insertion: triangles claimed for the next round
+    C_STN_T = 24,     // crossing pass: triangles in use
+    C_STN_S = 25,     // crossing pass: crossing vertices
     C_NUM = 32
 };
 
@@ -327,8 +329,23 @@ struct ConsResult {
 // nmax: constraint endpoints must be vertex ids in [0, nmax)
 // cell_end (optional): the Morton cell ends of m.pts (morton_order), so the
 // split pass finds the vertices on each segment by cells instead of walking
+// With `defer`, a piece whose walk meets a constrained edge (it crosses a
+// piece recovered before it) is retired instead of failing the solve; the
+// pieces and a flag per deferred piece are handed back for the crossing pass.
+struct ConsDefer {
+    DBuf<int2> pieces;
+    DBuf<int> flag;
+    int np = 0;
+};
 ConsResult insert_constraints(Ctx& c, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C, int* touched,
-                              const int* cell_end);
+                              const int* cell_end, ConsDefer* defer = nullptr);
+// Crossing pass (steiner.hip): inserts the deferred pieces into the CDT of the
+// others with a vertex at every crossing (ids nv0, nv0 + 1, ...); the mesh
+// arrays hold tcap triangles.  The crossing points are written to out with
+// their two pieces mapped to caller ids through perm (ids below n).  Returns
+// the new triangle count; *S receives the number of crossing vertices.
+int steiner_pass(Ctx& c, const Mesh& m, int T, int tcap, int nv0, const ConsDefer& d, int scap, gcdt_hpoint* out,
+                 const int* perm, int n, int* S);
 // mark every edge chain of the mesh that runs exactly along a constraint
 // (constraints split at collinear vertices) as constrained; returns pieces
 long long mark_constraint_chains(Ctx& c, const Mesh& m, int T, int nv, int nmax, const int2* cons, int C);
@@ -338,10 +355,17 @@ void write_corners(Ctx& c, int2* pts, int n);
 // the whole solve (gcdt_solve / gcdt_solve_flags) on device arrays
 struct SolveOut {
     int T = 0;
+    int S = 0;  // crossing vertices (gcdt_solve_steiner)
     gcdt_stats st{};
 };
+// crossing constraints resolved (gcdt_solve_steiner): at most scap crossing
+// vertices, written to out; the mesh arrays hold 2 (N + scap) + 2 triangles
+struct SteinerArgs {
+    int scap = 0;
+    gcdt_hpoint* out = nullptr;
+};
 void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, int3* tris_out, int3* adj_out,
-                SolveOut& out, int* cflag_out = nullptr);
+                SolveOut& out, int* cflag_out = nullptr, const SteinerArgs* steiner = nullptr);
 // set-up of gcdt_solve_noalloc on the current device (from gcdt_pool_init)
 void noalloc_setup();
 

@@ -53,6 +53,12 @@ class BatchStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class HPoint(ctypes.Structure):
+    """gcdt_hpoint: a crossing point (X / w, Y / w), X and Y 128-bit."""
+    _fields_ = [("x_hi", ctypes.c_int64), ("x_lo", ctypes.c_uint64), ("y_hi", ctypes.c_int64),
+                ("y_lo", ctypes.c_uint64), ("w", ctypes.c_int64), ("seg", ctypes.c_int32 * 4)]
+
+
 vp = ctypes.c_void_p
 pvp = ctypes.POINTER(ctypes.c_void_p)
 pint = ctypes.POINTER(ctypes.c_int)
@@ -108,6 +114,7 @@ SIGNATURES = {
     "gcdt_solve_flags": ([vp, i32, vp, i32, vp, vp, vp, pint, ctypes.POINTER(Stats), vp], i32),
     "gcdt_solve_noalloc": ([vp, i32, vp, i32, vp, vp, pint, ctypes.POINTER(Stats), vp], i32),
     "gcdt_carve": ([vp, vp, vp, i32, i32, vp, pint, vp], i32),
+    "gcdt_solve_steiner": ([vp, i32, vp, i32, i32, vp, vp, vp, vp, pint, pint, ctypes.POINTER(Stats), vp], i32),
     "gcdt_solve_batch": ([vp, ctypes.POINTER(ctypes.c_int64), i32, vp, ctypes.POINTER(ctypes.c_int64), vp,
                           ctypes.POINTER(BatchStats), vp, vp], i32),
     "gcdt_set_tunable": ([ctypes.c_char_p, i32], i32),

@@ -223,6 +223,34 @@ int gcdt_solve_flags(const gcdt_int2* points, int N, const gcdt_int2* cons, int 
 int gcdt_solve_noalloc(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, gcdt_int3* tris_out,
                        gcdt_int3* adj_out, int* num_tris, gcdt_stats* stats, void* stream);
 
+/* A crossing point of two constraint segments, exactly: (X / w, Y / w) with
+ * X = x_hi * 2^64 + x_lo and Y likewise (128-bit two's complement), w > 0.
+ * On the 2^30 grid |X|, |Y| < 2^92 and w < 2^61.  seg holds the vertex ids of
+ * two constraint pieces through it, (seg[0], seg[1]) and (seg[2], seg[3]). */
+typedef struct {
+    int64_t x_hi;
+    uint64_t x_lo;
+    int64_t y_hi;
+    uint64_t y_lo;
+    int64_t w;
+    int32_t seg[4];
+} gcdt_hpoint;
+
+/* gcdt_solve_flags with crossing constraints resolved instead of refused
+ * (GCDT_ERR_CROSSING): no counterpart in the reference, whose walk drops a
+ * segment that meets a constrained edge (addConstraints.cu:768-774,942-946).
+ * Every proper crossing of two constraint pieces becomes a vertex (ids N+4,
+ * N+5, ...; coordinates in steiner_out[0..*num_steiner)) and each piece a
+ * chain of constrained edges through its crossings; the result is the CDT of
+ * the points, the corners and the crossings.  tris_out, adj_out and cflag_out
+ * hold 2 (N + max_steiner) + 2 entries (adj_out and cflag_out may be NULL);
+ * *num_tris = 2 (N + *num_steiner) + 2.  More than max_steiner crossings fail
+ * with GCDT_ERR_CAPACITY.  The crossings are inserted by one device thread,
+ * one piece after another: meant for inputs with few crossings. */
+int gcdt_solve_steiner(const gcdt_int2* points, int N, const gcdt_int2* cons, int C, int max_steiner,
+                       gcdt_int3* tris_out, gcdt_int3* adj_out, int* cflag_out, gcdt_hpoint* steiner_out,
+                       int* num_tris, int* num_steiner, gcdt_stats* stats, void* stream);
+
 /* Even-odd carving (no counterpart in the reference, which keeps every
  * triangle of the bounding square): inside[t] = 1 for the triangles enclosed
  * by an odd number of constraint boundaries -- polygon interiors minus their

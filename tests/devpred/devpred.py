@@ -50,6 +50,9 @@ def host():
             f.restype = ctypes.c_int
         L.hostdev_mul_s64.argtypes = [vp, ctypes.c_int, vp, vp]
         L.hostdev_mul_s64.restype = ctypes.c_int
+        for f in (L.hostdev_orient_h, L.hostdev_incircle_h, L.hostdev_cross):
+            f.argtypes = [vp, ctypes.c_int, vp]
+            f.restype = ctypes.c_int
         _host = L
     return _host
 
@@ -66,6 +69,9 @@ def lib():
             f.restype = ctypes.c_int
         L.devpred_mul_s64.argtypes = [vp, ctypes.c_int, vp, vp]
         L.devpred_mul_s64.restype = ctypes.c_int
+        for f in (L.devpred_orient_h, L.devpred_incircle_h, L.devpred_cross):
+            f.argtypes = [vp, ctypes.c_int, vp]
+            f.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -93,3 +99,44 @@ def mul_s64(pairs, on_host=False):
     f = host().hostdev_mul_s64 if on_host else lib().devpred_mul_s64
     assert f(a.ctypes.data, len(a), lo.ctypes.data, hi.ctypes.data) == 0
     return lo, hi
+
+
+def _words(h):
+    """(X, Y, W) Python ints -> the 5 int64 words of a homogeneous point."""
+    x, y, w = (int(v) for v in h)
+    m = (1 << 64) - 1
+
+    def s64(v):
+        v &= m
+        return v - (1 << 64) if v >> 63 else v
+    return [s64(x), s64(x >> 64), s64(y), s64(y >> 64), int(w)]
+
+
+def orient_h(triples, on_host=False) -> np.ndarray:
+    a = np.array([[w for h in t for w in _words(h)] for t in triples], np.int64).reshape(-1, 15)
+    out = np.zeros(len(a), np.int32)
+    f = host().hostdev_orient_h if on_host else lib().devpred_orient_h
+    assert f(a.ctypes.data, len(a), out.ctypes.data) == 0
+    return out
+
+
+def incircle_h(quads, on_host=False) -> np.ndarray:
+    a = np.array([[w for h in t for w in _words(h)] for t in quads], np.int64).reshape(-1, 20)
+    out = np.zeros(len(a), np.int32)
+    f = host().hostdev_incircle_h if on_host else lib().devpred_incircle_h
+    assert f(a.ctypes.data, len(a), out.ctypes.data) == 0
+    return out
+
+
+def cross(segs, on_host=False) -> list:
+    """segs: (n, 8) grid coordinates p1 p2 q1 q2 -> [(X, Y, W)] (W 0: parallel)."""
+    a = np.ascontiguousarray(segs, np.int32).reshape(-1, 8)
+    out = np.zeros((len(a), 5), np.int64)
+    f = host().hostdev_cross if on_host else lib().devpred_cross
+    assert f(a.ctypes.data, len(a), out.ctypes.data) == 0
+    res = []
+    for o in out.tolist():
+        x = (o[1] << 64) | (o[0] & ((1 << 64) - 1))
+        y = (o[3] << 64) | (o[2] & ((1 << 64) - 1))
+        res.append((x, y, o[4]))
+    return res

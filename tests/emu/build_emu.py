@@ -22,8 +22,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "gcdt_amd", "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libgcdt_emu.so")
-SOURCES = ["mesh.hip", "cons.hip", "carve.hip", "io.hip", "noalloc.hip", "capi.cpp"]
-HEADERS = ["gcdt_common.hpp", "predicates.hpp", "mesh_bodies.hpp", "cons_bodies.hpp", "mesh.hpp"]
+SOURCES = ["mesh.hip", "cons.hip", "carve.hip", "io.hip", "noalloc.hip", "steiner.hip", "capi.cpp"]
+HEADERS = ["gcdt_common.hpp", "predicates.hpp", "mesh_bodies.hpp", "cons_bodies.hpp", "mesh.hpp", "exact.hpp", "steiner_bodies.hpp"]
 CXX = os.environ.get("CXX_EMU", "g++")
 FLAGS = ["-std=c++17", "-O1", "-g", "-fPIC", "-Wno-unknown-pragmas", "-Wno-attributes", "-fno-strict-aliasing",
          "-I", HERE]

@@ -121,6 +121,32 @@ def solve(g, cons, entry="gcdt_solve"):
     return tris, adj, st.as_dict()
 
 
+def solve_steiner(g, cons, max_steiner=None):
+    """gcdt_solve_steiner: (tris, [(X, Y, W)], stats)."""
+    from gcdt_amd.api import HPOINT_DTYPE, hpoint_coords
+    g = np.ascontiguousarray(g, np.int32).reshape(-1, 2)
+    c = np.ascontiguousarray(cons, np.int32).reshape(-1, 2)
+    n = len(g)
+    ms = 4 * len(c) + 64 if max_steiner is None else max_steiner
+    cap = 2 * (n + ms) + 2
+    dp, dt, da, df = Buf.of(g), Buf(12 * cap), Buf(12 * cap), Buf(4 * cap)
+    dh = Buf(HPOINT_DTYPE.itemsize * max(ms, 1))
+    dc = Buf.of(c) if len(c) else None
+    nt, ns = ctypes.c_int(0), ctypes.c_int(0)
+    st = L.Stats()
+    try:
+        check(load().gcdt_solve_steiner(dp.ptr, n, dc.ptr if dc else None, len(c), ms, dt.ptr, da.ptr, df.ptr,
+                                        dh.ptr, ctypes.byref(nt), ctypes.byref(ns), ctypes.byref(st), None),
+              "gcdt_solve_steiner")
+        tris = dt.get(np.int32, (nt.value, 3))
+        raw = dh.get(HPOINT_DTYPE, (ns.value,))
+    finally:
+        for b in (dp, dt, da, df, dh, dc):
+            if b:
+                b.free()
+    return tris, [hpoint_coords(h) for h in raw], st.as_dict()
+
+
 def solve_batch(instances):
     from gcdt_amd.api import pack_instances
     flat_p, poff, flat_c, coff, toff = pack_instances(instances)
@@ -188,6 +214,28 @@ def run_case(case: dict) -> dict:
             res["valid"] &= bool(v["valid"]) and bool(O.validate(g, t0, c)["valid"])
             if v["valid"] and v["unique"]:
                 res["unique_mismatch"] += int(not np.array_equal(O.canonical(t0), O.canonical(t1)))
+        return res
+    if kind == "steiner":
+        from tests import steiner_cases as SC
+        spec = case["gen"]
+        if spec[0] == "random":
+            g, c = SC.random_crossing(*spec[1:])
+        elif spec[0] == "star":
+            g, c = SC.star(*spec[1:])
+        else:
+            g, c = SC.hd_family(*spec[1:])
+        try:
+            tris, hp, st = solve_steiner(g, c, case.get("max_steiner"))
+        except L.GcdtError as e:
+            res["status"] = e.status
+            return res
+        res["steiner"] = len(hp)
+        try:
+            v = SC.check(g, c, tris, hp, brute=case.get("brute", False), augmented=case.get("augmented", False))
+            res["unique"] = bool(v["unique"])
+        except AssertionError as e:
+            res["valid"] = False
+            res["error"] = str(e)[:2000]
         return res
     if kind == "batch":
         inst = _instances(case)

@@ -13,6 +13,7 @@ import json
 import os
 import subprocess
 import sys
+import zlib
 
 import pytest
 
@@ -50,7 +51,7 @@ def _run(case, **kw):
 @pytest.mark.parametrize("knob,value", [(k, v) for k, vs in SWEEP.items() for v in vs])
 def test_batch_at_every_swept_value(knob, value):
     _run({"kind": "batch", "sizes": BATCH, "seed": 7, "knobs": {knob: value}},
-         seed=hash((knob, value)) % 1000 + 1, align_low=value & 1, conc=8, yield_atomics=1, yield_mem=2)
+         seed=zlib.crc32(f"{knob}={value}".encode()) % 1000 + 1, align_low=value & 1, conc=8, yield_atomics=1, yield_mem=2)
 
 
 def test_batch_all_knobs_off_default():
@@ -116,3 +117,29 @@ def test_noalloc_solve_refuses_without_setup_and_above_its_maximum():
     assert rc == 0 and out["status"] == [2], (rc, out, err)
     rc, out, err = emu.run_in_child({"kind": "noalloc", "inputs": [["uniform", 800000, 0, 54]]})
     assert rc == 0 and out["status"] == [6], (rc, out, err)
+
+
+# ---------------------------------------------------------------- crossing constraints
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_steiner_tiny_inputs_equal_the_brute_force_cdt(seed):
+    """Crossing pieces deferred by the cavity rounds (workgroups interleaved
+    by the emulator seed) and inserted by the serial pass: the exact
+    validator, and the brute-force CDT with the crossing vertices."""
+    out = _run({"kind": "steiner", "gen": ["random", seed, 12, 6], "brute": True}, seed=seed, conc=8,
+               yield_atomics=1)
+    assert out.get("status") is None
+
+
+@pytest.mark.parametrize("gen", [["star", 6, 4, 5], ["hd", 8, 5, 6, 40], ["random", 9, 400, 40]])
+def test_steiner_star_grid_crossings_and_many_crossings(gen):
+    """Four segments through one point; 30 grid crossings equal to the
+    oracle's CDT of the points plus the crossings; hundreds of crossings."""
+    out = _run({"kind": "steiner", "gen": gen, "augmented": gen[0] == "hd", "max_steiner": 2000}, seed=3, conc=8,
+               yield_atomics=1)
+    assert out["steiner"] > 0
+
+
+def test_steiner_capacity_is_reported():
+    rc, out, err = emu.run_in_child({"kind": "steiner", "gen": ["random", 9, 400, 40], "max_steiner": 3})
+    assert rc == 0 and out["status"] == 8, (out, err)
