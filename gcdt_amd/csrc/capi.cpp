@@ -289,7 +289,7 @@ void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, i
     // ---- 1. sample + sample mesh
     Timer t1(c.st);
     DBuf<int> sid(1 << 16);
-    int S = sample_points_morton(c, cell_end.p, sid.p);
+    int S = sample_points_morton(c, pts, cell_end.p, sid.p);
     // cell_end stays for the constraints' split pass
     DBuf<int> loc((size_t)N + 4), pend((size_t)(N > 1 ? N : 1)), pend2((size_t)(N > 1 ? N : 1));
     DBuf<unsigned long long> pkey((size_t)N + 4);

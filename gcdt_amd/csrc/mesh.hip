@@ -768,7 +768,14 @@ void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv
 
 // Sample of Morton-ordered points: the 256x256 sample cell with Morton index
 // m covers the 16 consecutive 20-bit cells [16m, 16m+16), i.e. the sorted
-// range [end[16m-1], end[16m+15]); its first point is the cell's sample.
+// range [end[16m-1], end[16m+15]).  Its sample is its point with the smallest
+// (x, y) in lexicographic order.  Any point of the cell serves location; the
+// extreme one does not depend on the order inside a 20-bit cell (which comes
+// from atomics), and on collinear rows and columns -- a lattice, or points
+// clamped to the square's edge -- it puts the samples on the same lines, so
+// the points between two samples split the edge joining them instead of
+// fanning out from a sample off the line, which Lawson flips then unwind
+// one row per round.
 __device__ inline int sample_range_start(const int* cell_end, int m)
 {
     return m ? cell_end[16 * m - 1] : 0;
@@ -780,21 +787,46 @@ __global__ __launch_bounds__(kBlock) void k_sample_morton_flags(const int* cell_
         flags[i] = cell_end[16 * i + 15] > sample_range_start(cell_end, i);
 }
 
-__global__ __launch_bounds__(kBlock) void k_sample_morton_emit(const int* cell_end, const int* offs, int* ids)
+// one wave per sample cell: the smallest (x, y) over the cell's sorted range
+__global__ __launch_bounds__(kBlock) void k_sample_morton_emit(const int2* pts, const int* cell_end, const int* offs,
+                                                               int* ids)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < SAMPLE_CELLS; i += gridDim.x * blockDim.x) {
-        int a = sample_range_start(cell_end, i);
-        if (cell_end[16 * i + 15] > a) ids[offs[i]] = a;
+    const int lane = __lane_id();
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; m < SAMPLE_CELLS; m += nwaves) {
+        const int a = sample_range_start(cell_end, m), b = cell_end[16 * m + 15];
+        if (b <= a) continue;  // wave-uniform
+        unsigned long long best = ~0ull;
+        int bi = a;
+        for (int i = a + lane; i < b; i += 64) {
+            const int2 p = pts[i];
+            const unsigned long long k = ((unsigned long long)(unsigned)p.x << 32) | (unsigned)p.y;
+            if (k < best) {
+                best = k;
+                bi = i;
+            }
+        }
+        unsigned long long w = best;
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl(w, lane ^ off, 64);
+            w = o < w ? o : w;
+        }
+        // points are distinct: exactly one lane holds the minimum
+        const unsigned long long owner = __ballot(best == w);
+        const int src = __ffsll((long long)owner) - 1;
+        const int id = __shfl(bi, src, 64);
+        if (lane == 0) ids[offs[m]] = id;
     }
 }
 
-int sample_points_morton(Ctx& c, const int* cell_end, int* sample_ids)
+int sample_points_morton(Ctx& c, const int2* pts, const int* cell_end, int* sample_ids)
 {
     DBuf<int> offs(SAMPLE_CELLS + 1);
     k_sample_morton_flags<<<grid_for(SAMPLE_CELLS), kBlock, 0, c.st>>>(cell_end, offs.p);
     GCDT_LAUNCH_CHECK();
     scan_excl(c, offs.p, offs.p, SAMPLE_CELLS);
-    k_sample_morton_emit<<<grid_for(SAMPLE_CELLS), kBlock, 0, c.st>>>(cell_end, offs.p, sample_ids);
+    // one wave per sample cell
+    k_sample_morton_emit<<<grid_for(64LL * SAMPLE_CELLS), kBlock, 0, c.st>>>(pts, cell_end, offs.p, sample_ids);
     GCDT_LAUNCH_CHECK();
     return c.get(C_TOTAL);
 }

@@ -261,8 +261,9 @@ long long flip_rounds_records(Ctx& c, FRec* rec, const Mesh& m, int T, int* roun
 // receives each cell's end offset in the sorted order.
 void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv, int* cell_end);
 // One sample point per occupied 256x256 cell of Morton-ordered points (the
-// first of the cell's range), from morton_order's cell_end; returns the count.
-int sample_points_morton(Ctx& c, const int* cell_end, int* sample_ids);
+// cell's point with the smallest (x, y)), from morton_order's cell_end;
+// returns the count.
+int sample_points_morton(Ctx& c, const int2* pts, const int* cell_end, int* sample_ids);
 void relabel_pairs(Ctx& c, const int2* in, int C, const int* inv, int n, int2* out);
 void unmap_tris(Ctx& c, int3* tri, int T, const int* perm, int n);
 // batches: instance map, per-instance corners/seed triangles/location, local
