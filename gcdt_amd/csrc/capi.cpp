@@ -312,7 +312,7 @@ void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, i
     int ntA = insert_rounds(c, ia, pend, pend2, S, 2, &rs);
     long long capB = 4LL * N + 8;
     DBuf<int3> tB(capB), aB(capB);
-    int Ts = filter_leaves(c, tA.p, aA.p, sA.p, ntA, tB.p, aB.p, capB);
+    int Ts = filter_leaves_morton(c, pts, tA.p, aA.p, sA.p, ntA, tB.p, aB.p, capB);
     tA.release();
     aA.release();
     sA.release();
@@ -866,7 +866,8 @@ void batch_impl(const gcdt_int2* points, const std::vector<long long>& poff, con
     long long capB = 2LL * K + 2LL * S + 4LL * (NP - S) + 8;
     if (capB > (1LL << 31) - 1) throw Error(GCDT_ERR_INVALID, "batch too large");
     DBuf<int3> tB(capB), aB(capB);
-    int Ts = filter_leaves(c, tA.p, aA.p, sA.p, ntA, tB.p, aB.p, capB);
+    int Ts = filter_leaves_inst(c, pts.p, vinst.p, d_cbase.p, d_lvl.p, ncells, NP, tA.p, aA.p, sA.p, ntA, tB.p,
+                                aB.p, capB);
     tA.release();
     aA.release();
     sA.release();
@@ -903,7 +904,8 @@ void batch_impl(const gcdt_int2* points, const std::vector<long long>& poff, con
     pend2.release();
     long long Tn = 2 * NP + 2LL * K;
     DBuf<FRec> rec(Tn);
-    int T = filter_leaves_inst_rec(c, pts.p, vinst.p, NP, K, tB.p, aB.p, sB.p, nt, rec.p, Tn);
+    int T = filter_leaves_inst_rec(c, pts.p, vinst.p, d_cbase.p, d_lvl.p, ncells, NP, tB.p, aB.p, sB.p, nt, rec.p,
+                                   Tn);
     tB.release();
     aB.release();
     sB.release();

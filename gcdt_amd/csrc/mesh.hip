@@ -731,9 +731,148 @@ __device__ inline unsigned morton_cell(int2 p)
     return x | (y << 1);
 }
 
-// Counting sort in two passes with one atomic per point: the count pass keeps
-// the value its atomicAdd returns (the point's rank within its cell) in
-// inv[], and the scatter places the point at the cell's start plus that rank.
+// ---------------------------------------------------------------- order inside cells
+
+// Counting sorts place the elements of one bucket in the order their atomics
+// arrived.  Where that order reaches a decision (point ids key the insertion
+// claims, triangle ids key the flip claims), each bucket's range is then
+// reordered by a key of its own elements -- the input index of a point, the
+// sorted vertex ids of a triangle -- so the order is the same on every run.
+// A bucket of up to SEG_RANK_MAX (64) elements is ranked by one lane per
+// element (its key against every key of the bucket, read at neighbouring
+// positions), up to SEG_BLOCK_MAX by one workgroup's bitonic sort in LDS;
+// larger ones keep the arrival order.
+#ifndef GCDT_SEG_RANK_MAX
+#define GCDT_SEG_RANK_MAX 64
+#endif
+#ifndef GCDT_SEG_BLOCK_MAX
+#define GCDT_SEG_BLOCK_MAX 2048
+#endif
+constexpr int SEG_RANK_MAX = GCDT_SEG_RANK_MAX;
+constexpr int SEG_BLOCK_MAX = GCDT_SEG_BLOCK_MAX;
+static_assert(SEG_RANK_MAX <= SEG_BLOCK_MAX, "bucket caps");
+
+// keys: a point's input index (the element itself)
+struct SegKeyIndex {
+    struct Key {
+        int a;
+    };
+    __device__ Key at(const int* idx, long long q) const { return Key{idx[q]}; }
+    __device__ static Key max_key() { return Key{INT_MAX}; }
+    __device__ static bool less(const Key& x, const Key& y) { return x.a < y.a; }
+};
+
+// keys: a triangle's vertex ids, smallest first, stored per position
+struct SegKeyTri {
+    const int3* key;
+    using Key = int3;
+    __device__ Key at(const int*, long long q) const { return key[q]; }
+    __device__ static Key max_key() { return make_int3(INT_MAX, INT_MAX, INT_MAX); }
+    __device__ static bool less(const Key& x, const Key& y)
+    {
+        return x.x < y.x || (x.x == y.x && (x.y < y.y || (x.y == y.y && x.z < y.z)));
+    }
+};
+
+__device__ inline int3 tri_sorted_ids(int3 t)
+{
+    const int a = min(t.x, min(t.y, t.z)), c = max(t.x, max(t.y, t.z));
+    return make_int3(a, t.x + t.y + t.z - a - c, c);
+}
+
+// One lane per position s; bkt[s] is the bucket of the element at s, whose
+// range is [end[b-1], end[b]) (end: inclusive scan).  Writes the element's
+// place in its bucket's key order to out[]; a bucket's first lane lists a
+// bucket for the workgroup pass.
+template <class KF>
+__global__ __launch_bounds__(kBlock) void k_seg_rank(KF kf, const int* end, const int* bkt, long long n, const int* idx,
+                                                     int* out, int* big, int* nbig)
+{
+    WAVE_LOOP(s, n)
+    {
+        bool list = false;
+        int b = 0;
+        if (s < n) {
+            b = bkt[s];
+            const long long a = b ? end[b - 1] : 0, k = end[b] - a;
+            long long pos = s;
+            if (k > 1 && k <= SEG_RANK_MAX) {
+                const typename KF::Key mine = kf.at(idx, s);
+                long long r = 0;
+                for (long long q = a; q < a + k; q++) r += KF::less(kf.at(idx, q), mine) ? 1 : 0;
+                pos = a + r;
+            }
+            out[pos] = idx[s];
+            list = s == a && k > SEG_RANK_MAX && k <= SEG_BLOCK_MAX;
+        }
+        const int p = wave_alloc(list ? 1 : 0, nbig);
+        if (list) big[p] = b;
+    }
+}
+
+// one workgroup per listed bucket of SEG_RANK_MAX + 1 .. SEG_BLOCK_MAX
+// elements: a bitonic sort in LDS, padded to a power of two with largest keys,
+// of the bucket's range of out[] (keys read from the same positions of idx)
+template <class KF>
+__global__ __launch_bounds__(kBlock) void k_seg_reorder_block(KF kf, const int* end, const int* big, const int* nbig,
+                                                              const int* idx, int* out)
+{
+    using Key = typename KF::Key;
+    __shared__ int se[SEG_BLOCK_MAX];
+    __shared__ Key sk[SEG_BLOCK_MAX];
+    const int nb = *nbig;
+    for (int q = blockIdx.x; q < nb; q += gridDim.x) {
+        const int b = big[q];
+        const int a = b ? end[b - 1] : 0, k = end[b] - a;
+        int np2 = 1;
+        while (np2 < k) np2 <<= 1;
+        for (int i = threadIdx.x; i < np2; i += kBlock) {
+            se[i] = i < k ? idx[a + i] : -1;
+            sk[i] = i < k ? kf.at(idx, a + i) : KF::max_key();
+        }
+        for (int size = 2; size <= np2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                __syncthreads();
+                for (int i = threadIdx.x; i < (np2 >> 1); i += kBlock) {
+                    const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                    const bool up = (lo & size) == 0;
+                    if (KF::less(sk[hi], sk[lo]) == up) {
+                        const Key tk = sk[lo];
+                        sk[lo] = sk[hi];
+                        sk[hi] = tk;
+                        const int te = se[lo];
+                        se[lo] = se[hi];
+                        se[hi] = te;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < k; i += kBlock) out[a + i] = se[i];
+        __syncthreads();
+    }
+}
+
+// idx[0..n): elements grouped by bucket; bkt[s]: the bucket of position s.
+// The reordered elements go to out[0..n).
+template <class KF>
+static void seg_reorder(Ctx& c, const KF& kf, const int* end, const int* bkt, long long n, const int* idx, int* out)
+{
+    if (n <= 0) return;
+    // a list entry per bucket above the rank cap: at most n / (cap + 1)
+    DBuf<int> big((size_t)(n / (SEG_RANK_MAX + 1) + 1));
+    c.set(C_NQ, 0);
+    k_seg_rank<KF><<<grid_for(n), kBlock, 0, c.st>>>(kf, end, bkt, n, idx, out, big.p, c.dcnt.p + C_NQ);
+    GCDT_LAUNCH_CHECK();
+    k_seg_reorder_block<KF><<<1024, kBlock, 0, c.st>>>(kf, end, big.p, c.dcnt.p + C_NQ, idx, out);
+    GCDT_LAUNCH_CHECK();
+}
+
+// Counting sort of points by 20-bit Morton cell: the count pass keeps the
+// value its atomicAdd returns (the point's arrival rank in its cell) in
+// inv[], the scatter places the point at the cell's start plus that rank,
+// each cell is reordered by input index, and the points and the inverse map
+// are written in that order.
 __global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n, int* cnt, int* rank)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
@@ -742,16 +881,23 @@ __global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n,
 
 // cell_end holds the inclusive scan of the counts: cell c starts at
 // cell_end[c - 1] (0 for c = 0).
-__global__ __launch_bounds__(kBlock) void k_morton_scatter(const int2* pts, int n, const int* cell_end, int2* out,
-                                                           int* perm, int* inv)
+__global__ __launch_bounds__(kBlock) void k_morton_scatter(const int2* pts, int n, const int* cell_end, int* perm,
+                                                           const int* rank, int* bkt)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        int2 p = pts[i];
-        int cell = (int)morton_cell(p);
-        int slot = (cell ? cell_end[cell - 1] : 0) + inv[i];
-        out[slot] = p;
-        perm[slot] = (int)i;
-        inv[i] = slot;
+        const int cell = (int)morton_cell(pts[i]);
+        const int pos = (cell ? cell_end[cell - 1] : 0) + rank[i];
+        perm[pos] = (int)i;
+        bkt[pos] = cell;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton_place(const int2* pts, int n, const int* perm, int2* out, int* inv)
+{
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (long long)gridDim.x * blockDim.x) {
+        const int i = perm[s];
+        out[s] = pts[i];
+        inv[i] = (int)s;
     }
 }
 
@@ -762,20 +908,27 @@ void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv
     k_morton_count<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv);
     GCDT_LAUNCH_CHECK();
     scan_incl(c, cell_end, cell_end, BITMAP_CELLS);
-    k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, out, perm, inv);
+    {
+        DBuf<int> bkt((size_t)n), arrived((size_t)n);
+        k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, arrived.p, inv, bkt.p);
+        GCDT_LAUNCH_CHECK();
+        seg_reorder(c, SegKeyIndex{}, cell_end, bkt.p, n, arrived.p, perm);
+    }
+    k_morton_place<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, perm, out, inv);
     GCDT_LAUNCH_CHECK();
 }
 
 // Sample of Morton-ordered points: the 256x256 sample cell with Morton index
 // m covers the 16 consecutive 20-bit cells [16m, 16m+16), i.e. the sorted
-// range [end[16m-1], end[16m+15]).  Its sample is its point with the smallest
-// (x, y) in lexicographic order.  Any point of the cell serves location; the
-// extreme one does not depend on the order inside a 20-bit cell (which comes
-// from atomics), and on collinear rows and columns -- a lattice, or points
-// clamped to the square's edge -- it puts the samples on the same lines, so
-// the points between two samples split the edge joining them instead of
-// fanning out from a sample off the line, which Lawson flips then unwind
-// one row per round.
+// range [end[16m-1], end[16m+15]).  Its sample is its extreme point towards
+// the nearer edges of the square: the smallest (x, y) in lexicographic order,
+// with x taken as -x in the right half of the square and y as -y in the top
+// half.  Any point of the cell serves location; the extreme one does not
+// depend on the order of the sorted range, and on collinear rows and columns
+// -- a lattice, or points clamped to the square's edge -- it puts the samples
+// on the outermost lines, so the points between two samples split the edge
+// joining them instead of fanning out from a sample off the line (a fan that
+// Lawson flips unwind one row per round).
 __device__ inline int sample_range_start(const int* cell_end, int m)
 {
     return m ? cell_end[16 * m - 1] : 0;
@@ -787,7 +940,7 @@ __global__ __launch_bounds__(kBlock) void k_sample_morton_flags(const int* cell_
         flags[i] = cell_end[16 * i + 15] > sample_range_start(cell_end, i);
 }
 
-// one wave per sample cell: the smallest (x, y) over the cell's sorted range
+// one wave per sample cell: its extreme point over the cell's sorted range
 __global__ __launch_bounds__(kBlock) void k_sample_morton_emit(const int2* pts, const int* cell_end, const int* offs,
                                                                int* ids)
 {
@@ -796,11 +949,18 @@ __global__ __launch_bounds__(kBlock) void k_sample_morton_emit(const int2* pts, 
     for (int m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; m < SAMPLE_CELLS; m += nwaves) {
         const int a = sample_range_start(cell_end, m), b = cell_end[16 * m + 15];
         if (b <= a) continue;  // wave-uniform
+        // the cell's column and row (even and odd bits of its Morton index)
+        unsigned cx = 0, cy = 0;
+        for (int bit = 0; bit < 8; bit++) {
+            cx |= (((unsigned)m >> (2 * bit)) & 1u) << bit;
+            cy |= (((unsigned)m >> (2 * bit + 1)) & 1u) << bit;
+        }
+        const unsigned fx = cx >= 128 ? ~0u : 0u, fy = cy >= 128 ? ~0u : 0u;
         unsigned long long best = ~0ull;
         int bi = a;
         for (int i = a + lane; i < b; i += 64) {
             const int2 p = pts[i];
-            const unsigned long long k = ((unsigned long long)(unsigned)p.x << 32) | (unsigned)p.y;
+            const unsigned long long k = ((unsigned long long)((unsigned)p.x ^ fx) << 32) | ((unsigned)p.y ^ fy);
             if (k < best) {
                 best = k;
                 bi = i;
@@ -1931,32 +2091,45 @@ __global__ __launch_bounds__(kBlock) void k_batch_cons(const int2* in, const lon
     }
 }
 
-// scatter the global mesh back into per-instance lists with local vertex ids
+// scatter the global mesh back into per-instance lists with local vertex ids.
+// Leaves are in vertex order and point ids instance-major, so an instance's
+// triangles fill one range of slots [first, last] (flips and cavities reuse
+// slots in place): a triangle's place in its list is its slot minus first,
+// the same on every run.  A range that does not hold exactly the instance's
+// triangle count flags MERR_INTERNAL_COUNT.
+__device__ inline int batch_inst_of(int3 t, const int* vinst, long long NP)
+{
+    return t.x < NP ? vinst[t.x] : (int)((t.x - NP) >> 2);
+}
+
+// the ends of each instance's run of slots, where the instance changes (one
+// store per end; a split run stores an end twice and fails the count check)
+__global__ __launch_bounds__(kBlock) void k_batch_span(const int3* tri, int T, const int* vinst, long long NP,
+                                                       int* first, int* last)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        const int inst = batch_inst_of(tri[i], vinst, NP);
+        if (i == 0 || batch_inst_of(tri[i - 1], vinst, NP) != inst) first[inst] = (int)i;
+        if (i == T - 1 || batch_inst_of(tri[i + 1], vinst, NP) != inst) last[inst] = (int)i;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_batch_out(const int3* tri, int T, const int* vinst, const int* perm,
                                                       const long long* poff, const long long* toff, long long NP,
-                                                      int* cursor, int3* out, int* err)
+                                                      const int* first, const int* last, int3* out, int* err)
 {
-    WAVE_LOOP(i, T)
-    {
-        int3 t = make_int3(0, 0, 0);
-        int inst = -1;
-        if (i < T) {
-            t = tri[i];
-            inst = t.x < NP ? vinst[t.x] : (int)((t.x - NP) >> 2);
-        }
-        // triangles of one instance sit together in the mesh: one atomic per run
-        int r = wave_run_add(cursor, inst);
-        if (inst < 0) continue;
-        long long base = poff[inst], n = poff[inst + 1] - poff[inst];
-        auto local = [&](int v) -> int {
-            return v < NP ? (int)((perm ? perm[v] : v) - base) : (int)(n + ((v - NP) & 3));
-        };
-        long long pos = toff[inst] + r;
-        if (pos >= toff[inst + 1]) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        const int3 t = tri[i];
+        const int inst = batch_inst_of(t, vinst, NP);
+        const long long base = poff[inst], n = poff[inst + 1] - poff[inst];
+        if ((long long)last[inst] - first[inst] + 1 != toff[inst + 1] - toff[inst]) {
             atomicOr(err, MERR_INTERNAL_COUNT);
             continue;
         }
-        out[pos] = make_int3(local(t.x), local(t.y), local(t.z));
+        auto local = [&](int v) -> int {
+            return v < NP ? (int)((perm ? perm[v] : v) - base) : (int)(n + ((v - NP) & 3));
+        };
+        out[toff[inst] + (i - first[inst])] = make_int3(local(t.x), local(t.y), local(t.z));
     }
 }
 
@@ -1985,6 +2158,8 @@ struct BatchCells {
     }
 };
 
+// points by (instance, fine cell), each cell in input order (the count,
+// scatter and per-cell reorder of the single solve's Morton order)
 __global__ __launch_bounds__(kBlock) void k_batch_cell_count(BatchCells b, const int2* pts, long long NP, int* cnt,
                                                              int* rank)
 {
@@ -1993,15 +2168,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_cell_count(BatchCells b, const
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_cell_scatter(BatchCells b, const int2* pts, long long NP,
-                                                               const int* cell_end, int2* out, int* perm, int* inv)
+                                                               const int* cell_end, int* perm, const int* rank,
+                                                               int* bkt)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
-        int2 p = pts[i];
-        int cell = b.cell((int)i, p);
-        int slot = (cell ? cell_end[cell - 1] : 0) + inv[i];
-        out[slot] = p;
-        perm[slot] = (int)i;
-        inv[i] = slot;
+        const int cell = b.cell((int)i, pts[i]);
+        const int pos = (cell ? cell_end[cell - 1] : 0) + rank[i];
+        perm[pos] = (int)i;
+        bkt[pos] = cell;
     }
 }
 
@@ -2014,7 +2188,13 @@ void batch_order(Ctx& c, const int2* pts, const int* vinst, const int* cbase, co
     k_batch_cell_count<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv);
     GCDT_LAUNCH_CHECK();
     scan_incl(c, cell_end, cell_end, ncells);
-    k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, out, perm, inv);
+    {
+        DBuf<int> bkt((size_t)NP), arrived((size_t)NP);
+        k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, arrived.p, inv, bkt.p);
+        GCDT_LAUNCH_CHECK();
+        seg_reorder(c, SegKeyIndex{}, cell_end, bkt.p, NP, arrived.p, perm);
+    }
+    k_morton_place<<<grid_for(NP), kBlock, 0, c.st>>>(pts, (int)NP, perm, out, inv);
     GCDT_LAUNCH_CHECK();
 }
 
@@ -2095,9 +2275,12 @@ void batch_cons(Ctx& c, const int2* in, const long long* coff, const long long* 
 void batch_out(Ctx& c, const int3* tri, int T, const int* vinst, const int* perm, const long long* poff,
                const long long* toff, long long NP, int K, int3* out)
 {
-    DBuf<int> cursor(K);
-    fill_i32(c, cursor.p, 0, K);
-    k_batch_out<<<grid_for(T), kBlock, 0, c.st>>>(tri, T, vinst, perm, poff, toff, NP, cursor.p, out,
+    DBuf<int> first(K), last(K);
+    fill_i32(c, first.p, INT_MAX, K);
+    fill_i32(c, last.p, -1, K);
+    k_batch_span<<<grid_for(T), kBlock, 0, c.st>>>(tri, T, vinst, NP, first.p, last.p);
+    GCDT_LAUNCH_CHECK();
+    k_batch_out<<<grid_for(T), kBlock, 0, c.st>>>(tri, T, vinst, perm, poff, toff, NP, first.p, last.p, out,
                                                   c.dcnt.p + C_ERR);
     GCDT_LAUNCH_CHECK();
     c.check_err();
@@ -2154,8 +2337,20 @@ __global__ __launch_bounds__(kBlock) void k_leaf_scatter(const int3* tri, const 
 // compaction): neighbouring triangles end up near each other in memory.
 // Sort key of a leaf for the compacted mesh order: the Morton cell of its
 // centroid on the 1024x1024 grid over the square.
+#ifndef GCDT_LEAF_CELL_BITS
+#define GCDT_LEAF_CELL_BITS 10
+#endif
+constexpr int LEAF_CELL_BITS = GCDT_LEAF_CELL_BITS;
+constexpr long long LEAF_CELLS = 1LL << (2 * LEAF_CELL_BITS);
+__device__ inline unsigned morton_cell_leaf(int2 p)
+{
+    unsigned x = (unsigned)p.x >> (30 - LEAF_CELL_BITS), y = (unsigned)p.y >> (30 - LEAF_CELL_BITS);
+    unsigned r = 0;
+    for (int b = 0; b < LEAF_CELL_BITS; b++) r |= (((x >> b) & 1u) << (2 * b)) | (((y >> b) & 1u) << (2 * b + 1));
+    return r;
+}
 struct LeafKeyOne {
-    __device__ int key(const int3&, int2 g) const { return (int)morton_cell(g); }
+    __device__ int key(const int3&, int2 g) const { return (int)morton_cell_leaf(g); }
 };
 
 // The count pass keeps each leaf's rank within its cell in rank[]; the dest
@@ -2184,14 +2379,26 @@ __global__ __launch_bounds__(kBlock) void k_leaf_cells(KEY kf, const int2* pts, 
 
 // dest[] holds the ranks on entry and the destinations on exit; src[] gets
 // the inverse (the leaf that lands at each destination).
-__global__ __launch_bounds__(kBlock) void k_leaf_dest(const int* sons, const int* cellof, int T, const int* start,
-                                                      int* dest, int* src, long long cap)
+__global__ __launch_bounds__(kBlock) void k_leaf_dest(const int3* tri, const int* sons, const int* cellof, int T,
+                                                      const int* start, int* dest, int* src, int* bkt, int3* key,
+                                                      long long cap)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
         int d = sons[i] == -1 ? start[cellof[i]] + dest[i] : -1;
         dest[i] = d;
-        if (d >= 0 && d < cap) src[d] = (int)i;
+        if (d >= 0 && d < cap) {
+            src[d] = (int)i;
+            bkt[d] = cellof[i];
+            key[d] = tri_sorted_ids(tri[i]);
+        }
     }
+}
+
+// dest[] from the reordered src[]
+__global__ __launch_bounds__(kBlock) void k_leaf_redest(const int* src, int n, int* dest)
+{
+    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x)
+        dest[src[o]] = (int)o;
 }
 
 // Where the moved leaves go: plain tri/adj arrays, or straight into the
@@ -2240,11 +2447,21 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
     fill_i32(c, cnt.p, 0, ncells + 1);
     k_leaf_cells<KEY><<<grid_for(T), kBlock, 0, c.st>>>(kf, pts, tri, sons, T, cnt.p, cellof.p, dest.p);
     GCDT_LAUNCH_CHECK();
-    scan_excl(c, cnt.p, cnt.p, ncells);
+    scan_excl_total(c, cnt.p, cnt.p, ncells);
     int n = c.get(C_TOTAL);
     if (n > cap) throw Error(GCDT_ERR_CAPACITY, "more leaves than the output holds");
     DBuf<int> src(n > 0 ? n : 1);
-    k_leaf_dest<<<grid_for(T), kBlock, 0, c.st>>>(sons, cellof.p, T, cnt.p, dest.p, src.p, cap);
+    {
+        DBuf<int> arrived(n > 0 ? n : 1), bkt(n > 0 ? n : 1);
+        DBuf<int3> key(n > 0 ? n : 1);
+        k_leaf_dest<<<grid_for(T), kBlock, 0, c.st>>>(tri, sons, cellof.p, T, cnt.p, dest.p, arrived.p, bkt.p, key.p,
+                                                      cap);
+        GCDT_LAUNCH_CHECK();
+        // each cell's leaves by their vertex ids (the count pass placed them
+        // in arrival order); cnt holds the cells' starts, cnt[c + 1] their ends
+        seg_reorder(c, SegKeyTri{key.p}, cnt.p + 1, bkt.p, n, arrived.p, src.p);
+    }
+    k_leaf_redest<<<grid_for(n), kBlock, 0, c.st>>>(src.p, n, dest.p);
     GCDT_LAUNCH_CHECK();
     if (n > 0) {
         ProfScope ps("leaf_move", c.st, n);
@@ -2257,38 +2474,43 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
 int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2,
                          int3* adj2, long long cap)
 {
-    return filter_leaves_sorted(c, LeafKeyOne{}, BITMAP_CELLS, pts, tri, adj, sons, T, LeafToArrays{tri2, adj2}, cap);
+    return filter_leaves_sorted(c, LeafKeyOne{}, LEAF_CELLS, pts, tri, adj, sons, T, LeafToArrays{tri2, adj2}, cap);
 }
 
 int filter_leaves_morton_rec(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T,
                              FRec* rec, long long cap)
 {
-    return filter_leaves_sorted(c, LeafKeyOne{}, BITMAP_CELLS, pts, tri, adj, sons, T, LeafToRecords{rec, pts}, cap);
+    return filter_leaves_sorted(c, LeafKeyOne{}, LEAF_CELLS, pts, tri, adj, sons, T, LeafToRecords{rec, pts}, cap);
 }
 
-// Batch meshes: instance-major, then the Morton order of the centroid's
-// 16x16 cell over the instance's square.
+// Batch meshes: instance-major, then the Morton order of the centroid's cell
+// in the instance's own grid of 4^L cells (the point order's cells, about one
+// point each), so a cell holds a few leaves and its reorder stays in one wave.
 struct LeafKeyInst {
     const int* vinst;
+    const int* cbase;
+    const int* lvl;
     long long NP;  // corners of instance i are NP + 4i .. NP + 4i + 3
     __device__ int key(const int3& t, int2 g) const
     {
-        int inst = t.x < NP ? vinst[t.x] : (int)((t.x - NP) >> 2);
-        return (inst << 8) | (int)(morton_cell(g) >> 12);
+        const int inst = t.x < NP ? vinst[t.x] : (int)((t.x - NP) >> 2);
+        return cbase[inst] + (int)(morton_cell(g) >> (2 * (10 - lvl[inst])));
     }
 };
 
-int filter_leaves_inst(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
-                       const int3* adj, const int* sons, int T, int3* tri2, int3* adj2, long long cap)
+int filter_leaves_inst(Ctx& c, const int2* pts, const int* vinst, const int* cbase, const int* lvl, long long ncells,
+                       long long NP, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2, int3* adj2,
+                       long long cap)
 {
-    return filter_leaves_sorted(c, LeafKeyInst{vinst, NP}, (long long)K << 8, pts, tri, adj, sons, T,
+    return filter_leaves_sorted(c, LeafKeyInst{vinst, cbase, lvl, NP}, ncells, pts, tri, adj, sons, T,
                                 LeafToArrays{tri2, adj2}, cap);
 }
 
-int filter_leaves_inst_rec(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
-                           const int3* adj, const int* sons, int T, FRec* rec, long long cap)
+int filter_leaves_inst_rec(Ctx& c, const int2* pts, const int* vinst, const int* cbase, const int* lvl,
+                           long long ncells, long long NP, const int3* tri, const int3* adj, const int* sons, int T,
+                           FRec* rec, long long cap)
 {
-    return filter_leaves_sorted(c, LeafKeyInst{vinst, NP}, (long long)K << 8, pts, tri, adj, sons, T,
+    return filter_leaves_sorted(c, LeafKeyInst{vinst, cbase, lvl, NP}, ncells, pts, tri, adj, sons, T,
                                 LeafToRecords{rec, pts}, cap);
 }
 

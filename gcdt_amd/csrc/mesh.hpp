@@ -261,8 +261,8 @@ long long flip_rounds_records(Ctx& c, FRec* rec, const Mesh& m, int T, int* roun
 // receives each cell's end offset in the sorted order.
 void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv, int* cell_end);
 // One sample point per occupied 256x256 cell of Morton-ordered points (the
-// cell's point with the smallest (x, y)), from morton_order's cell_end;
-// returns the count.
+// cell's extreme point towards the nearer edges of the square), from
+// morton_order's cell_end; returns the count.
 int sample_points_morton(Ctx& c, const int2* pts, const int* cell_end, int* sample_ids);
 void relabel_pairs(Ctx& c, const int2* in, int C, const int* inv, int n, int2* out);
 void unmap_tris(Ctx& c, int3* tri, int T, const int* perm, int n);
@@ -298,14 +298,17 @@ int filter_leaves(Ctx& c, const int3* tri, const int3* adj, const int* sons, int
 // same, with the leaves placed in Morton order of their centroid cell
 int filter_leaves_morton(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2,
                          int3* adj2, long long cap);
-// batch meshes: leaves sorted by (instance, 16x16 centroid cell)
-int filter_leaves_inst(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
-                       const int3* adj, const int* sons, int T, int3* tri2, int3* adj2, long long cap);
+// batch meshes: leaves sorted by (instance, centroid cell in the instance's
+// grid of 4^L cells: cbase/lvl/ncells as for the batched point order)
+int filter_leaves_inst(Ctx& c, const int2* pts, const int* vinst, const int* cbase, const int* lvl, long long ncells,
+                       long long NP, const int3* tri, const int3* adj, const int* sons, int T, int3* tri2, int3* adj2,
+                       long long cap);
 // the same two orders, written straight into packed flip records
 int filter_leaves_morton_rec(Ctx& c, const int2* pts, const int3* tri, const int3* adj, const int* sons, int T,
                              FRec* rec, long long cap);
-int filter_leaves_inst_rec(Ctx& c, const int2* pts, const int* vinst, long long NP, int K, const int3* tri,
-                           const int3* adj, const int* sons, int T, FRec* rec, long long cap);
+int filter_leaves_inst_rec(Ctx& c, const int2* pts, const int* vinst, const int* cbase, const int* lvl,
+                           long long ncells, long long NP, const int3* tri, const int3* adj, const int* sons, int T,
+                           FRec* rec, long long cap);
 // bitmap CSR over 1024^2 cells; allocates offsets/tris from the pool
 void build_bitmap(Ctx& c, const int2* pts, const int3* tri, int T, int** offsets, int** tris, int* ntris);
 // mark vertices of tri[0..T) in isv (must be zeroed), then locate the other

@@ -104,12 +104,26 @@ __host__ __device__ inline unsigned long long ins_key(int p, int onedge, unsigne
            ((unsigned long long)(close & 2047u) << 32) | (unsigned)p;
 }
 
+// A point on the edge opposite slot k < 3 is ranked against that edge alone:
+// its two triangles may record it in either, and both must give one key.
+__host__ __device__ inline unsigned closeness_rank_edge(int2 U, int2 V, int2 P, int p)
+{
+    long long dx = 2LL * P.x - ((long long)U.x + V.x);
+    long long dy = 2LL * P.y - ((long long)U.y + V.y);
+    long long d = (dx < 0 ? -dx : dx) + (dy < 0 ? -dy : dy);
+    long long size = 2LL * ((U.x > V.x ? (long long)U.x - V.x : (long long)V.x - U.x) +
+                            (U.y > V.y ? (long long)U.y - V.y : (long long)V.y - U.y));
+    unsigned cls = d * 8 < size ? 3u : (d * 4 < size ? 2u : (d * 2 < size ? 1u : 0u));
+    return (cls << 9) | (hash32((uint32_t)p) & 511u);
+}
+
 __host__ __device__ inline unsigned long long ins_key_of(const Ins& s, int p, int t, int k, unsigned epoch)
 {
     unsigned close;
     if (s.pivot == 1) {
         int3 tv = s.m.tri[t];
-        close = closeness_rank(s.m.pts[tv.x], s.m.pts[tv.y], s.m.pts[tv.z], s.m.pts[p], p);
+        close = k < 3 ? closeness_rank_edge(s.m.pts[get3(tv, nx3(k))], s.m.pts[get3(tv, pv3(k))], s.m.pts[p], p)
+                      : closeness_rank(s.m.pts[tv.x], s.m.pts[tv.y], s.m.pts[tv.z], s.m.pts[p], p);
     } else {
         close = hash32((uint32_t)p) & 2047u;
     }

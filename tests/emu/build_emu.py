@@ -88,7 +88,7 @@ def rewrite_launches(s: str) -> str:
     return "".join(out)
 
 
-_SHARED = re.compile(r"__shared__\s+((?:unsigned\s+)?(?:long\s+long|int|char|short)|int[234])\s+([^;]+);")
+_SHARED = re.compile(r"__shared__\s+((?:unsigned\s+)?(?:long\s+long|int|char|short)|int[234]|[A-Za-z_][\w:]*)\s+([^;]+);")
 _lds_id = [0]
 
 
@@ -115,8 +115,11 @@ def _compile(src_path: str, obj: str, extra=()):
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    gen = os.path.join(OUT, "src")
+def build(verbose: bool = False, defines=(), lib: str = LIB) -> str:
+    """defines: extra -D macros (a variant library, built in its own folder)."""
+    out = OUT if not defines else os.path.join(OUT, "v_" + "_".join(d.replace("=", "") for d in defines))
+    extra = [f"-D{d}" for d in defines]
+    gen = os.path.join(out, "src")
     os.makedirs(gen, exist_ok=True)
     os.makedirs(os.path.join(OUT, "include"), exist_ok=True)
     # headers are copied unchanged beside the rewritten sources (relative
@@ -135,19 +138,19 @@ def build(verbose: bool = False) -> str:
         dst = os.path.join(gen, os.path.splitext(src)[0] + ".cpp")
         with open(dst, "w") as f:
             f.write(rewrite_shared(rewrite_launches(txt)))
-        jobs.append((dst, os.path.join(OUT, os.path.splitext(src)[0] + ".o")))
+        jobs.append((dst, os.path.join(out, os.path.splitext(src)[0] + ".o"), extra))
     rt = os.path.join(HERE, "emu_rt.cpp")
-    jobs.append((rt, os.path.join(OUT, "emu_rt.o"), ["-I", gen]))
+    jobs.append((rt, os.path.join(out, "emu_rt.o"), ["-I", gen, *extra]))
     with ThreadPoolExecutor(max_workers=min(6, len(jobs))) as ex:
         objs = list(ex.map(lambda j: _compile(*j), jobs))
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     r = subprocess.run([CXX, "-shared", "-o", tmp, *objs, "-lpthread"], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"emulator link failed:\n{r.stderr[-4000:]}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", lib)
+    return lib
 
 
 if __name__ == "__main__":

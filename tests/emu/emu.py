@@ -27,30 +27,38 @@ import importlib  # noqa: E402
 L = importlib.import_module("gcdt_amd.lib")  # signature table only; libgcdt.so is not loaded
 
 LIB = os.path.join(HERE, "_build", "libgcdt_emu.so")
+# variant whose per-cell reorder ranks buckets of up to 3 elements lane by
+# lane, sorts 4-8 in a workgroup and keeps the order of larger ones, so all
+# three paths run on small meshes (the defaults are 64 and 2048)
+SEGCAP = (("GCDT_SEG_RANK_MAX=3", "GCDT_SEG_BLOCK_MAX=8"),
+          os.path.join(HERE, "_build", "libgcdt_emu_segcap.so"))
 _lib = None
 
 
-def build():
-    """Builds the emulator library unless it is newer than every source
-    (serialised by a lock file, so parallel test workers build it once)."""
+def build(variant=None):
+    """Builds the emulator library (or a variant: (defines, path)) unless it
+    is newer than every source (serialised by a lock file, so parallel test
+    workers build it once)."""
     import fcntl
     from tests.emu import build_emu
+    defines, lib = variant if variant else ((), LIB)
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
     with open(os.path.join(HERE, "_build", ".lock"), "w") as lk:
         fcntl.flock(lk, fcntl.LOCK_EX)
         srcs = [os.path.join(build_emu.CSRC, f) for f in build_emu.SOURCES + build_emu.HEADERS]
         srcs += [os.path.join(HERE, f) for f in ("emu_rt.cpp", "build_emu.py", os.path.join("hip", "hip_runtime.h"))]
-        if os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(f) for f in srcs):
-            return LIB
-        return build_emu.build()
+        if os.path.exists(lib) and os.path.getmtime(lib) >= max(os.path.getmtime(f) for f in srcs):
+            return lib
+        return build_emu.build(defines=defines, lib=lib)
 
 
 def load():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        path = os.environ.get("EMU_LIB") or LIB
+        if not os.path.exists(path):
             build()
-        d = ctypes.CDLL(LIB)
+        d = ctypes.CDLL(path)
         for name, (args, res) in L.SIGNATURES.items():
             f = getattr(d, name, None)
             if f is None:
@@ -240,6 +248,9 @@ def run_case(case: dict) -> dict:
     if kind == "batch":
         inst = _instances(case)
         tris, st = solve_batch(inst)
+        if case.get("digest"):
+            import hashlib
+            res["digest"] = hashlib.sha256(b"".join(t.tobytes() for t in tris)).hexdigest()
         bad = []
         for i, ((g, c), t) in enumerate(zip(inst, tris)):
             v = O.validate(g, t, c)
@@ -260,6 +271,9 @@ def run_case(case: dict) -> dict:
             raise ValueError(kind)
         g = gen.to_grid(p)
         tris, adj, st = solve(g, c)
+        if case.get("digest"):
+            import hashlib
+            res["digest"] = hashlib.sha256(tris.tobytes() + adj.tobytes()).hexdigest()
         v = O.validate(g, tris, c)
         res["valid"] = bool(v["valid"])
         if v["valid"] and v.get("unique"):
@@ -271,7 +285,7 @@ def run_case(case: dict) -> dict:
 
 
 def run_in_child(case: dict, seed: int = 1, align_low: int = 0, conc: int = 1, yield_atomics: int = 0,
-                 timeout: float = 600.0, yield_mem: int = 0):
+                 timeout: float = 600.0, yield_mem: int = 0, lib: str | None = None):
     """Runs `case` in a child process; returns (exit status, parsed result or
     None, stderr tail).  seed: order of workgroups and work-items and the
     fill of new allocations; align_low: allocations against their lower guard
@@ -280,6 +294,8 @@ def run_in_child(case: dict, seed: int = 1, align_low: int = 0, conc: int = 1, y
     of a record load or store."""
     env = dict(os.environ, EMU_SEED=str(seed), EMU_ALIGN_LOW=str(align_low), EMU_CONC=str(conc),
                EMU_YIELD_ATOMICS=str(yield_atomics), EMU_YIELD_MEM=str(yield_mem))
+    if lib:
+        env["EMU_LIB"] = lib
     r = subprocess.run([sys.executable, "-m", "tests.emu.emu", json.dumps(case)], cwd=ROOT, env=env,
                        capture_output=True, text=True, timeout=timeout)
     out = None

@@ -143,3 +143,33 @@ def test_steiner_star_grid_crossings_and_many_crossings(gen):
 def test_steiner_capacity_is_reported():
     rc, out, err = emu.run_in_child({"kind": "steiner", "gen": ["random", 9, 400, 40], "max_steiner": 3})
     assert rc == 0 and out["status"] == 8, (out, err)
+
+
+# ---------------------------------------------------------------- determinism
+
+@pytest.mark.parametrize("case", [{"kind": "lattice", "k": 96, "c": 12, "max_len": 20},
+                                  {"kind": "clustered", "n": 30000, "circles": 10},
+                                  {"kind": "batch", "sizes": [64, 300, 2000, 900, 150, 20000], "seed": 7}],
+                         ids=["lattice", "clustered", "batch"])
+def test_same_output_under_every_interleaving(case):
+    """Triangle and adjacency arrays are bit-identical whatever order the
+    emulated workgroups and work-items run in (atomics arrive in a different
+    order each time), degenerate lattice and batches included."""
+    digests = set()
+    for seed, conc in ((1, 1), (2, 8), (3, 16)):
+        out = _run(dict(case, digest=1), seed=seed, conc=conc, yield_atomics=1)
+        digests.add(out["digest"])
+    assert len(digests) == 1
+
+
+def test_per_cell_reorder_paths_keep_valid_meshes():
+    """With the per-cell reorder's caps at 3 and 8 elements, the lane-rank,
+    the workgroup and the keep-order paths all run on small meshes: every
+    result must still pass the validator (a reorder that lost or doubled an
+    element would corrupt the point or leaf order)."""
+    lib = emu.build(emu.SEGCAP)
+    for case in ({"kind": "lattice", "k": 64, "c": 8, "max_len": 12}, {"kind": "uniform", "n": 6000, "c": 40},
+                 {"kind": "clustered", "n": 8000, "circles": 4},
+                 {"kind": "batch", "sizes": [64, 700, 300], "seed": 9}):
+        rc, out, err = emu.run_in_child(case, seed=5, conc=8, yield_atomics=1, lib=lib)
+        assert rc == 0 and out is not None and out["valid"], (case, out, err)
