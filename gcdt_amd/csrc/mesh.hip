@@ -738,10 +738,12 @@ __device__ inline unsigned morton_cell(int2 p)
 // claims, triangle ids key the flip claims), each bucket's range is then
 // reordered by a key of its own elements -- the input index of a point, the
 // sorted vertex ids of a triangle -- so the order is the same on every run.
-// A bucket of up to SEG_RANK_MAX (64) elements is ranked by one lane per
-// element (its key against every key of the bucket, read at neighbouring
-// positions), up to SEG_BLOCK_MAX by one workgroup's bitonic sort in LDS;
-// larger ones keep the arrival order.
+// The scatter writes one int2 (element, bucket) per position.  A bucket of up
+// to SEG_RANK_MAX (64) elements is ranked by one lane per element, its key
+// against every key of the bucket (staged in LDS by the workgroup, read from
+// global memory only for the part of a bucket outside the workgroup's
+// positions); buckets of up to SEG_BLOCK_MAX by one workgroup's bitonic sort
+// in LDS; larger ones keep the arrival order.
 #ifndef GCDT_SEG_RANK_MAX
 #define GCDT_SEG_RANK_MAX 64
 #endif
@@ -757,16 +759,21 @@ struct SegKeyIndex {
     struct Key {
         int a;
     };
-    __device__ Key at(const int* idx, long long q) const { return Key{idx[q]}; }
+    __device__ Key of(int e) const { return Key{e}; }
     __device__ static Key max_key() { return Key{INT_MAX}; }
     __device__ static bool less(const Key& x, const Key& y) { return x.a < y.a; }
 };
 
-// keys: a triangle's vertex ids, smallest first, stored per position
+// keys: a triangle's vertex ids, smallest first
 struct SegKeyTri {
-    const int3* key;
+    const int3* tri;
     using Key = int3;
-    __device__ Key at(const int*, long long q) const { return key[q]; }
+    __device__ Key of(int e) const
+    {
+        const int3 t = tri[e];
+        const int a = min(t.x, min(t.y, t.z)), c = max(t.x, max(t.y, t.z));
+        return make_int3(a, t.x + t.y + t.z - a - c, c);
+    }
     __device__ static Key max_key() { return make_int3(INT_MAX, INT_MAX, INT_MAX); }
     __device__ static bool less(const Key& x, const Key& y)
     {
@@ -774,48 +781,57 @@ struct SegKeyTri {
     }
 };
 
-__device__ inline int3 tri_sorted_ids(int3 t)
-{
-    const int a = min(t.x, min(t.y, t.z)), c = max(t.x, max(t.y, t.z));
-    return make_int3(a, t.x + t.y + t.z - a - c, c);
-}
-
-// One lane per position s; bkt[s] is the bucket of the element at s, whose
-// range is [end[b-1], end[b]) (end: inclusive scan).  Writes the element's
-// place in its bucket's key order to out[]; a bucket's first lane lists a
-// bucket for the workgroup pass.
+// eb[s] = (element, bucket) at position s; a bucket's range is
+// [end[b-1], end[b]) (end: inclusive scan).  Writes each element of a bucket of
+// up to SEG_RANK_MAX at its place in the bucket's key order and every other
+// element where it is; a bucket's first lane lists a bucket of
+// SEG_RANK_MAX + 1 .. SEG_BLOCK_MAX for the workgroup pass.
 template <class KF>
-__global__ __launch_bounds__(kBlock) void k_seg_rank(KF kf, const int* end, const int* bkt, long long n, const int* idx,
-                                                     int* out, int* big, int* nbig)
+__global__ __launch_bounds__(kBlock) void k_seg_rank(KF kf, const int* end, const int2* eb, long long n, int* out,
+                                                     int* big, int* nbig)
 {
-    WAVE_LOOP(s, n)
-    {
-        bool list = false;
-        int b = 0;
+    using Key = typename KF::Key;
+    __shared__ Key sk[kBlock];
+    for (long long s0 = (long long)blockIdx.x * kBlock; s0 < n; s0 += (long long)gridDim.x * kBlock) {
+        const long long s = s0 + threadIdx.x;
+        int2 e = make_int2(0, 0);
+        long long a = 0, k = 0;
         if (s < n) {
-            b = bkt[s];
-            const long long a = b ? end[b - 1] : 0, k = end[b] - a;
+            e = eb[s];
+            a = e.y ? end[e.y - 1] : 0;
+            k = end[e.y] - a;
+            if (k > 1 && k <= SEG_RANK_MAX) sk[threadIdx.x] = kf.of(e.x);
+        }
+        __syncthreads();
+        bool list = false;
+        if (s < n) {
             long long pos = s;
             if (k > 1 && k <= SEG_RANK_MAX) {
-                const typename KF::Key mine = kf.at(idx, s);
+                // every position of the bucket inside [s0, s0 + kBlock) has
+                // staged its key (same bucket, same k)
+                const Key mine = sk[threadIdx.x];
                 long long r = 0;
-                for (long long q = a; q < a + k; q++) r += KF::less(kf.at(idx, q), mine) ? 1 : 0;
+                for (long long q = a; q < a + k; q++) {
+                    const long long t = q - s0;
+                    const Key o = (t >= 0 && t < kBlock) ? sk[t] : kf.of(eb[q].x);
+                    r += KF::less(o, mine) ? 1 : 0;
+                }
                 pos = a + r;
             }
-            out[pos] = idx[s];
+            out[pos] = e.x;
             list = s == a && k > SEG_RANK_MAX && k <= SEG_BLOCK_MAX;
         }
         const int p = wave_alloc(list ? 1 : 0, nbig);
-        if (list) big[p] = b;
+        if (list) big[p] = e.y;
+        __syncthreads();
     }
 }
 
-// one workgroup per listed bucket of SEG_RANK_MAX + 1 .. SEG_BLOCK_MAX
-// elements: a bitonic sort in LDS, padded to a power of two with largest keys,
-// of the bucket's range of out[] (keys read from the same positions of idx)
+// one workgroup per listed bucket: a bitonic sort in LDS, padded to a power of
+// two with largest keys, of the bucket's elements into its range of out[]
 template <class KF>
 __global__ __launch_bounds__(kBlock) void k_seg_reorder_block(KF kf, const int* end, const int* big, const int* nbig,
-                                                              const int* idx, int* out)
+                                                              const int2* eb, int* out)
 {
     using Key = typename KF::Key;
     __shared__ int se[SEG_BLOCK_MAX];
@@ -827,8 +843,9 @@ __global__ __launch_bounds__(kBlock) void k_seg_reorder_block(KF kf, const int* 
         int np2 = 1;
         while (np2 < k) np2 <<= 1;
         for (int i = threadIdx.x; i < np2; i += kBlock) {
-            se[i] = i < k ? idx[a + i] : -1;
-            sk[i] = i < k ? kf.at(idx, a + i) : KF::max_key();
+            const int el = i < k ? eb[a + i].x : -1;
+            se[i] = el;
+            sk[i] = i < k ? kf.of(el) : KF::max_key();
         }
         for (int size = 2; size <= np2; size <<= 1) {
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
@@ -853,18 +870,18 @@ __global__ __launch_bounds__(kBlock) void k_seg_reorder_block(KF kf, const int* 
     }
 }
 
-// idx[0..n): elements grouped by bucket; bkt[s]: the bucket of position s.
-// The reordered elements go to out[0..n).
+// eb[0..n): (element, bucket), grouped by bucket.  The reordered elements go
+// to out[0..n).
 template <class KF>
-static void seg_reorder(Ctx& c, const KF& kf, const int* end, const int* bkt, long long n, const int* idx, int* out)
+static void seg_reorder(Ctx& c, const KF& kf, const int* end, const int2* eb, long long n, int* out)
 {
     if (n <= 0) return;
     // a list entry per bucket above the rank cap: at most n / (cap + 1)
     DBuf<int> big((size_t)(n / (SEG_RANK_MAX + 1) + 1));
     c.set(C_NQ, 0);
-    k_seg_rank<KF><<<grid_for(n), kBlock, 0, c.st>>>(kf, end, bkt, n, idx, out, big.p, c.dcnt.p + C_NQ);
+    k_seg_rank<KF><<<grid_for(n), kBlock, 0, c.st>>>(kf, end, eb, n, out, big.p, c.dcnt.p + C_NQ);
     GCDT_LAUNCH_CHECK();
-    k_seg_reorder_block<KF><<<1024, kBlock, 0, c.st>>>(kf, end, big.p, c.dcnt.p + C_NQ, idx, out);
+    k_seg_reorder_block<KF><<<1024, kBlock, 0, c.st>>>(kf, end, big.p, c.dcnt.p + C_NQ, eb, out);
     GCDT_LAUNCH_CHECK();
 }
 
@@ -881,14 +898,13 @@ __global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n,
 
 // cell_end holds the inclusive scan of the counts: cell c starts at
 // cell_end[c - 1] (0 for c = 0).
-__global__ __launch_bounds__(kBlock) void k_morton_scatter(const int2* pts, int n, const int* cell_end, int* perm,
-                                                           const int* rank, int* bkt)
+__global__ __launch_bounds__(kBlock) void k_morton_scatter(const int2* pts, int n, const int* cell_end,
+                                                           const int* rank, int2* eb)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const int cell = (int)morton_cell(pts[i]);
         const int pos = (cell ? cell_end[cell - 1] : 0) + rank[i];
-        perm[pos] = (int)i;
-        bkt[pos] = cell;
+        eb[pos] = make_int2((int)i, cell);
     }
 }
 
@@ -909,10 +925,10 @@ void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv
     GCDT_LAUNCH_CHECK();
     scan_incl(c, cell_end, cell_end, BITMAP_CELLS);
     {
-        DBuf<int> bkt((size_t)n), arrived((size_t)n);
-        k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, arrived.p, inv, bkt.p);
+        DBuf<int2> eb((size_t)n);
+        k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv, eb.p);
         GCDT_LAUNCH_CHECK();
-        seg_reorder(c, SegKeyIndex{}, cell_end, bkt.p, n, arrived.p, perm);
+        seg_reorder(c, SegKeyIndex{}, cell_end, eb.p, n, perm);
     }
     k_morton_place<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, perm, out, inv);
     GCDT_LAUNCH_CHECK();
@@ -2168,14 +2184,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_cell_count(BatchCells b, const
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_cell_scatter(BatchCells b, const int2* pts, long long NP,
-                                                               const int* cell_end, int* perm, const int* rank,
-                                                               int* bkt)
+                                                               const int* cell_end, const int* rank, int2* eb)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
         const int cell = b.cell((int)i, pts[i]);
         const int pos = (cell ? cell_end[cell - 1] : 0) + rank[i];
-        perm[pos] = (int)i;
-        bkt[pos] = cell;
+        eb[pos] = make_int2((int)i, cell);
     }
 }
 
@@ -2189,10 +2203,10 @@ void batch_order(Ctx& c, const int2* pts, const int* vinst, const int* cbase, co
     GCDT_LAUNCH_CHECK();
     scan_incl(c, cell_end, cell_end, ncells);
     {
-        DBuf<int> bkt((size_t)NP), arrived((size_t)NP);
-        k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, arrived.p, inv, bkt.p);
+        DBuf<int2> eb((size_t)NP);
+        k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv, eb.p);
         GCDT_LAUNCH_CHECK();
-        seg_reorder(c, SegKeyIndex{}, cell_end, bkt.p, NP, arrived.p, perm);
+        seg_reorder(c, SegKeyIndex{}, cell_end, eb.p, NP, perm);
     }
     k_morton_place<<<grid_for(NP), kBlock, 0, c.st>>>(pts, (int)NP, perm, out, inv);
     GCDT_LAUNCH_CHECK();
@@ -2379,18 +2393,13 @@ __global__ __launch_bounds__(kBlock) void k_leaf_cells(KEY kf, const int2* pts, 
 
 // dest[] holds the ranks on entry and the destinations on exit; src[] gets
 // the inverse (the leaf that lands at each destination).
-__global__ __launch_bounds__(kBlock) void k_leaf_dest(const int3* tri, const int* sons, const int* cellof, int T,
-                                                      const int* start, int* dest, int* src, int* bkt, int3* key,
-                                                      long long cap)
+__global__ __launch_bounds__(kBlock) void k_leaf_dest(const int* sons, const int* cellof, int T, const int* start,
+                                                      int* dest, int2* eb, long long cap)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
         int d = sons[i] == -1 ? start[cellof[i]] + dest[i] : -1;
         dest[i] = d;
-        if (d >= 0 && d < cap) {
-            src[d] = (int)i;
-            bkt[d] = cellof[i];
-            key[d] = tri_sorted_ids(tri[i]);
-        }
+        if (d >= 0 && d < cap) eb[d] = make_int2((int)i, cellof[i]);
     }
 }
 
@@ -2452,14 +2461,13 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
     if (n > cap) throw Error(GCDT_ERR_CAPACITY, "more leaves than the output holds");
     DBuf<int> src(n > 0 ? n : 1);
     {
-        DBuf<int> arrived(n > 0 ? n : 1), bkt(n > 0 ? n : 1);
-        DBuf<int3> key(n > 0 ? n : 1);
-        k_leaf_dest<<<grid_for(T), kBlock, 0, c.st>>>(tri, sons, cellof.p, T, cnt.p, dest.p, arrived.p, bkt.p, key.p,
-                                                      cap);
+        DBuf<int2> eb(n > 0 ? n : 1);
+        k_leaf_dest<<<grid_for(T), kBlock, 0, c.st>>>(sons, cellof.p, T, cnt.p, dest.p, eb.p, cap);
         GCDT_LAUNCH_CHECK();
+        cellof.release();
         // each cell's leaves by their vertex ids (the count pass placed them
         // in arrival order); cnt holds the cells' starts, cnt[c + 1] their ends
-        seg_reorder(c, SegKeyTri{key.p}, cnt.p + 1, bkt.p, n, arrived.p, src.p);
+        seg_reorder(c, SegKeyTri{tri}, cnt.p + 1, eb.p, n, src.p);
     }
     k_leaf_redest<<<grid_for(n), kBlock, 0, c.st>>>(src.p, n, dest.p);
     GCDT_LAUNCH_CHECK();

@@ -43,13 +43,13 @@ import numpy as np
 # Device bytes a solve needs while it runs, from the MemCache pool's measured
 # peaks: the single solve peaked at 9,674,490,880 B for 10,099,763 points
 # with 100,000 segments (958 B per point, inputs and outputs included); the
-# batched solve of the 4,096-instance workload at 18,825,479,168 B for
-# 38,759,094 points (486 B per point: its sample meshes and insertion DAG are
+# batched solve of the 4,096-instance workload at 17,474,913,280 B for
+# 38,759,094 points (451 B per point: its sample meshes and insertion DAG are
 # per instance, the flip records shared).  Per call, the fixed buffers (cell
 # tables, bitmap, counters).
 SINGLE_BYTES_PER_POINT = 1024
 SINGLE_BYTES_PER_CALL = 64 << 20
-BATCH_BYTES_PER_POINT = 512
+BATCH_BYTES_PER_POINT = 480
 BATCH_BYTES_PER_CALL = 16 << 20
 # the triangles a finished request leaves on the device until it is fetched
 RESULT_BYTES_PER_POINT = 24

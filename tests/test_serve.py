@@ -37,12 +37,12 @@ def test_footprints_cover_the_measured_pool_peaks():
     (bench.py's pool_peak_bytes, inputs and outputs included): the single
     solve of the 10M workload (10,099,763 points, 100,000 segments) peaked at
     9,674,490,880 bytes in use, the batch of 4,096 instances (38,759,094
-    points) at 18,825,479,168."""
+    points) at 17,474,913,280."""
     assert S.footprint(10_099_763) >= 9_674_490_880
     from gcdt_amd import gen
     ns = [len(p) for p, _ in gen.batched_instances(count=4096)]
     assert sum(ns) == 38_759_094
-    assert S.batch_footprint(ns) >= 18_825_479_168
+    assert S.batch_footprint(ns) >= 17_474_913_280
 
 
 def test_results_hold_their_bytes_until_fetched():
