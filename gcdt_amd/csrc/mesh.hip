@@ -2350,9 +2350,10 @@ __global__ __launch_bounds__(kBlock) void k_leaf_scatter(const int3* tri, const 
 // reorderFaces, kernel.cu:1886-1964, done as a counting sort fused with the
 // compaction): neighbouring triangles end up near each other in memory.
 // Sort key of a leaf for the compacted mesh order: the Morton cell of its
-// centroid on the 1024x1024 grid over the square.
+// centroid on the 2048x2048 grid over the square (about five leaves per cell
+// at 10M points, so nearly every cell is ranked lane by lane).
 #ifndef GCDT_LEAF_CELL_BITS
-#define GCDT_LEAF_CELL_BITS 10
+#define GCDT_LEAF_CELL_BITS 11
 #endif
 constexpr int LEAF_CELL_BITS = GCDT_LEAF_CELL_BITS;
 constexpr long long LEAF_CELLS = 1LL << (2 * LEAF_CELL_BITS);
