@@ -739,13 +739,12 @@ __device__ inline unsigned morton_cell(int2 p)
 // reordered by a key of its own elements -- the input index of a point, the
 // sorted vertex ids of a triangle -- so the order is the same on every run.
 // The scatter writes one int2 (element, bucket) per position.  A bucket of up
-// to SEG_RANK_MAX (64) elements is ranked by one lane per element, its key
-// against every key of the bucket (staged in LDS by the workgroup, read from
-// global memory only for the part of a bucket outside the workgroup's
-// positions); buckets of up to SEG_BLOCK_MAX by one workgroup's bitonic sort
-// in LDS; larger ones keep the arrival order.
+// to SEG_RANK_MAX (256) elements is ranked by one lane per element, its key
+// against every key of the bucket (all staged in LDS by the workgroup); buckets
+// of up to SEG_BLOCK_MAX by one workgroup's bitonic sort in LDS; larger ones
+// keep the arrival order.
 #ifndef GCDT_SEG_RANK_MAX
-#define GCDT_SEG_RANK_MAX 64
+#define GCDT_SEG_RANK_MAX 256
 #endif
 #ifndef GCDT_SEG_BLOCK_MAX
 #define GCDT_SEG_BLOCK_MAX 2048
@@ -785,13 +784,17 @@ struct SegKeyTri {
 // [end[b-1], end[b]) (end: inclusive scan).  Writes each element of a bucket of
 // up to SEG_RANK_MAX at its place in the bucket's key order and every other
 // element where it is; a bucket's first lane lists a bucket of
-// SEG_RANK_MAX + 1 .. SEG_BLOCK_MAX for the workgroup pass.
+// SEG_RANK_MAX + 1 .. SEG_BLOCK_MAX for the workgroup pass.  The workgroup
+// first stages the keys of every position its rankable buckets cover (its own
+// kBlock positions and the parts of its edge buckets beyond them, at most
+// SEG_RANK_MAX - 1 on either side), so the rank loops read only LDS.
 template <class KF>
 __global__ __launch_bounds__(kBlock) void k_seg_rank(KF kf, const int* end, const int2* eb, long long n, int* out,
                                                      int* big, int* nbig)
 {
     using Key = typename KF::Key;
-    __shared__ Key sk[kBlock];
+    __shared__ Key sk[kBlock + 2 * (SEG_RANK_MAX - 1)];
+    __shared__ int s_lo, s_hi;
     for (long long s0 = (long long)blockIdx.x * kBlock; s0 < n; s0 += (long long)gridDim.x * kBlock) {
         const long long s = s0 + threadIdx.x;
         int2 e = make_int2(0, 0);
@@ -800,22 +803,31 @@ __global__ __launch_bounds__(kBlock) void k_seg_rank(KF kf, const int* end, cons
             e = eb[s];
             a = e.y ? end[e.y - 1] : 0;
             k = end[e.y] - a;
-            if (k > 1 && k <= SEG_RANK_MAX) sk[threadIdx.x] = kf.of(e.x);
         }
+        const bool rank = s < n && k > 1 && k <= SEG_RANK_MAX;
+        if (threadIdx.x == 0) {
+            s_lo = kBlock;
+            s_hi = 0;
+        }
+        __syncthreads();
+        // window [lo, hi) relative to s0: lo >= -(SEG_RANK_MAX - 1),
+        // hi <= kBlock + SEG_RANK_MAX - 1
+        if (rank) {
+            atomicMin(&s_lo, (int)(a - s0));
+            atomicMax(&s_hi, (int)(a + k - s0));
+        }
+        __syncthreads();
+        const int lo = s_lo, hi = s_hi;
+        for (int t = lo + (int)threadIdx.x; t < hi; t += kBlock) sk[t - lo] = kf.of(eb[s0 + t].x);
         __syncthreads();
         bool list = false;
         if (s < n) {
             long long pos = s;
-            if (k > 1 && k <= SEG_RANK_MAX) {
-                // every position of the bucket inside [s0, s0 + kBlock) has
-                // staged its key (same bucket, same k)
-                const Key mine = sk[threadIdx.x];
-                long long r = 0;
-                for (long long q = a; q < a + k; q++) {
-                    const long long t = q - s0;
-                    const Key o = (t >= 0 && t < kBlock) ? sk[t] : kf.of(eb[q].x);
-                    r += KF::less(o, mine) ? 1 : 0;
-                }
+            if (rank) {
+                const Key mine = sk[s - s0 - lo];
+                const Key* w = sk + (a - s0 - lo);
+                int r = 0;
+                for (int q = 0; q < (int)k; q++) r += KF::less(w[q], mine) ? 1 : 0;
                 pos = a + r;
             }
             out[pos] = e.x;
