@@ -739,22 +739,22 @@ __device__ inline unsigned morton_cell(int2 p)
 // reordered by a key of its own elements -- the input index of a point, the
 // sorted vertex ids of a triangle -- so the order is the same on every run.
 // The scatter writes one int2 (element, bucket) per position.  A bucket of up
-// to SEG_RANK_MAX (256) elements is ranked by one lane per element, its key
+// to CELL_RANK_MAX (256) elements is ranked by one lane per element, its key
 // against every key of the bucket (all staged in LDS by the workgroup); buckets
-// of up to SEG_BLOCK_MAX by one workgroup's bitonic sort in LDS; larger ones
+// of up to CELL_BLOCK_MAX by one workgroup's bitonic sort in LDS; larger ones
 // keep the arrival order.
-#ifndef GCDT_SEG_RANK_MAX
-#define GCDT_SEG_RANK_MAX 256
+#ifndef GCDT_CELL_RANK_MAX
+#define GCDT_CELL_RANK_MAX 256
 #endif
-#ifndef GCDT_SEG_BLOCK_MAX
-#define GCDT_SEG_BLOCK_MAX 2048
+#ifndef GCDT_CELL_BLOCK_MAX
+#define GCDT_CELL_BLOCK_MAX 2048
 #endif
-constexpr int SEG_RANK_MAX = GCDT_SEG_RANK_MAX;
-constexpr int SEG_BLOCK_MAX = GCDT_SEG_BLOCK_MAX;
-static_assert(SEG_RANK_MAX <= SEG_BLOCK_MAX, "bucket caps");
+constexpr int CELL_RANK_MAX = GCDT_CELL_RANK_MAX;
+constexpr int CELL_BLOCK_MAX = GCDT_CELL_BLOCK_MAX;
+static_assert(CELL_RANK_MAX <= CELL_BLOCK_MAX, "bucket caps");
 
 // keys: a point's input index (the element itself)
-struct SegKeyIndex {
+struct CellKeyIndex {
     struct Key {
         int a;
     };
@@ -764,7 +764,7 @@ struct SegKeyIndex {
 };
 
 // keys: a triangle's vertex ids, smallest first
-struct SegKeyTri {
+struct CellKeyTri {
     const int3* tri;
     using Key = int3;
     __device__ Key of(int e) const
@@ -782,18 +782,18 @@ struct SegKeyTri {
 
 // eb[s] = (element, bucket) at position s; a bucket's range is
 // [end[b-1], end[b]) (end: inclusive scan).  Writes each element of a bucket of
-// up to SEG_RANK_MAX at its place in the bucket's key order and every other
+// up to CELL_RANK_MAX at its place in the bucket's key order and every other
 // element where it is; a bucket's first lane lists a bucket of
-// SEG_RANK_MAX + 1 .. SEG_BLOCK_MAX for the workgroup pass.  The workgroup
+// CELL_RANK_MAX + 1 .. CELL_BLOCK_MAX for the workgroup pass.  The workgroup
 // first stages the keys of every position its rankable buckets cover (its own
 // kBlock positions and the parts of its edge buckets beyond them, at most
-// SEG_RANK_MAX - 1 on either side), so the rank loops read only LDS.
+// CELL_RANK_MAX - 1 on either side), so the rank loops read only LDS.
 template <class KF>
-__global__ __launch_bounds__(kBlock) void k_seg_rank(KF kf, const int* end, const int2* eb, long long n, int* out,
+__global__ __launch_bounds__(kBlock) void k_cell_rank(KF kf, const int* end, const int2* eb, long long n, int* out,
                                                      int* big, int* nbig)
 {
     using Key = typename KF::Key;
-    __shared__ Key sk[kBlock + 2 * (SEG_RANK_MAX - 1)];
+    __shared__ Key sk[kBlock + 2 * (CELL_RANK_MAX - 1)];
     __shared__ int s_lo, s_hi;
     for (long long s0 = (long long)blockIdx.x * kBlock; s0 < n; s0 += (long long)gridDim.x * kBlock) {
         const long long s = s0 + threadIdx.x;
@@ -804,14 +804,14 @@ __global__ __launch_bounds__(kBlock) void k_seg_rank(KF kf, const int* end, cons
             a = e.y ? end[e.y - 1] : 0;
             k = end[e.y] - a;
         }
-        const bool rank = s < n && k > 1 && k <= SEG_RANK_MAX;
+        const bool rank = s < n && k > 1 && k <= CELL_RANK_MAX;
         if (threadIdx.x == 0) {
             s_lo = kBlock;
             s_hi = 0;
         }
         __syncthreads();
-        // window [lo, hi) relative to s0: lo >= -(SEG_RANK_MAX - 1),
-        // hi <= kBlock + SEG_RANK_MAX - 1
+        // window [lo, hi) relative to s0: lo >= -(CELL_RANK_MAX - 1),
+        // hi <= kBlock + CELL_RANK_MAX - 1
         if (rank) {
             atomicMin(&s_lo, (int)(a - s0));
             atomicMax(&s_hi, (int)(a + k - s0));
@@ -831,7 +831,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_rank(KF kf, const int* end, cons
                 pos = a + r;
             }
             out[pos] = e.x;
-            list = s == a && k > SEG_RANK_MAX && k <= SEG_BLOCK_MAX;
+            list = s == a && k > CELL_RANK_MAX && k <= CELL_BLOCK_MAX;
         }
         const int p = wave_alloc(list ? 1 : 0, nbig);
         if (list) big[p] = e.y;
@@ -842,12 +842,12 @@ __global__ __launch_bounds__(kBlock) void k_seg_rank(KF kf, const int* end, cons
 // one workgroup per listed bucket: a bitonic sort in LDS, padded to a power of
 // two with largest keys, of the bucket's elements into its range of out[]
 template <class KF>
-__global__ __launch_bounds__(kBlock) void k_seg_reorder_block(KF kf, const int* end, const int* big, const int* nbig,
+__global__ __launch_bounds__(kBlock) void k_cell_sort_block(KF kf, const int* end, const int* big, const int* nbig,
                                                               const int2* eb, int* out)
 {
     using Key = typename KF::Key;
-    __shared__ int se[SEG_BLOCK_MAX];
-    __shared__ Key sk[SEG_BLOCK_MAX];
+    __shared__ int se[CELL_BLOCK_MAX];
+    __shared__ Key sk[CELL_BLOCK_MAX];
     const int nb = *nbig;
     for (int q = blockIdx.x; q < nb; q += gridDim.x) {
         const int b = big[q];
@@ -885,15 +885,15 @@ __global__ __launch_bounds__(kBlock) void k_seg_reorder_block(KF kf, const int* 
 // eb[0..n): (element, bucket), grouped by bucket.  The reordered elements go
 // to out[0..n).
 template <class KF>
-static void seg_reorder(Ctx& c, const KF& kf, const int* end, const int2* eb, long long n, int* out)
+static void cell_reorder(Ctx& c, const KF& kf, const int* end, const int2* eb, long long n, int* out)
 {
     if (n <= 0) return;
     // a list entry per bucket above the rank cap: at most n / (cap + 1)
-    DBuf<int> big((size_t)(n / (SEG_RANK_MAX + 1) + 1));
+    DBuf<int> big((size_t)(n / (CELL_RANK_MAX + 1) + 1));
     c.set(C_NQ, 0);
-    k_seg_rank<KF><<<grid_for(n), kBlock, 0, c.st>>>(kf, end, eb, n, out, big.p, c.dcnt.p + C_NQ);
+    k_cell_rank<KF><<<grid_for(n), kBlock, 0, c.st>>>(kf, end, eb, n, out, big.p, c.dcnt.p + C_NQ);
     GCDT_LAUNCH_CHECK();
-    k_seg_reorder_block<KF><<<1024, kBlock, 0, c.st>>>(kf, end, big.p, c.dcnt.p + C_NQ, eb, out);
+    k_cell_sort_block<KF><<<1024, kBlock, 0, c.st>>>(kf, end, big.p, c.dcnt.p + C_NQ, eb, out);
     GCDT_LAUNCH_CHECK();
 }
 
@@ -940,7 +940,7 @@ void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv
         DBuf<int2> eb((size_t)n);
         k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv, eb.p);
         GCDT_LAUNCH_CHECK();
-        seg_reorder(c, SegKeyIndex{}, cell_end, eb.p, n, perm);
+        cell_reorder(c, CellKeyIndex{}, cell_end, eb.p, n, perm);
     }
     k_morton_place<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, perm, out, inv);
     GCDT_LAUNCH_CHECK();
@@ -2218,7 +2218,7 @@ void batch_order(Ctx& c, const int2* pts, const int* vinst, const int* cbase, co
         DBuf<int2> eb((size_t)NP);
         k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv, eb.p);
         GCDT_LAUNCH_CHECK();
-        seg_reorder(c, SegKeyIndex{}, cell_end, eb.p, NP, perm);
+        cell_reorder(c, CellKeyIndex{}, cell_end, eb.p, NP, perm);
     }
     k_morton_place<<<grid_for(NP), kBlock, 0, c.st>>>(pts, (int)NP, perm, out, inv);
     GCDT_LAUNCH_CHECK();
@@ -2480,7 +2480,7 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
         cellof.release();
         // each cell's leaves by their vertex ids (the count pass placed them
         // in arrival order); cnt holds the cells' starts, cnt[c + 1] their ends
-        seg_reorder(c, SegKeyTri{tri}, cnt.p + 1, eb.p, n, src.p);
+        cell_reorder(c, CellKeyTri{tri}, cnt.p + 1, eb.p, n, src.p);
     }
     k_leaf_redest<<<grid_for(n), kBlock, 0, c.st>>>(src.p, n, dest.p);
     GCDT_LAUNCH_CHECK();

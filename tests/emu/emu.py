@@ -30,8 +30,8 @@ LIB = os.path.join(HERE, "_build", "libgcdt_emu.so")
 # variant whose per-cell reorder ranks buckets of up to 3 elements lane by
 # lane, sorts 4-8 in a workgroup and keeps the order of larger ones, so all
 # three paths run on small meshes (the defaults are 256 and 2048)
-SEGCAP = (("GCDT_SEG_RANK_MAX=3", "GCDT_SEG_BLOCK_MAX=8"),
-          os.path.join(HERE, "_build", "libgcdt_emu_segcap.so"))
+CELLCAP = (("GCDT_CELL_RANK_MAX=3", "GCDT_CELL_BLOCK_MAX=8"),
+          os.path.join(HERE, "_build", "libgcdt_emu_cellcap.so"))
 _lib = None
 
 

@@ -167,7 +167,7 @@ def test_per_cell_reorder_paths_keep_valid_meshes():
     the workgroup and the keep-order paths all run on small meshes: every
     result must still pass the validator (a reorder that lost or doubled an
     element would corrupt the point or leaf order)."""
-    lib = emu.build(emu.SEGCAP)
+    lib = emu.build(emu.CELLCAP)
     for case in ({"kind": "lattice", "k": 64, "c": 8, "max_len": 12}, {"kind": "uniform", "n": 6000, "c": 40},
                  {"kind": "clustered", "n": 8000, "circles": 4},
                  {"kind": "batch", "sizes": [64, 700, 300], "seed": 9}):
