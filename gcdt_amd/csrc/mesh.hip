@@ -741,8 +741,9 @@ __device__ inline unsigned morton_cell(int2 p)
 // The scatter writes one int2 (element, bucket) per position.  A bucket of up
 // to CELL_RANK_MAX (256) elements is ranked by one lane per element, its key
 // against every key of the bucket (all staged in LDS by the workgroup); buckets
-// of up to CELL_BLOCK_MAX by one workgroup's bitonic sort in LDS; larger ones
-// keep the arrival order.
+// of up to CELL_BLOCK_MAX by one workgroup's bitonic sort in LDS; larger ones,
+// which the count passes flag, by a merge sort of all their elements
+// (cell_reorder_huge).
 #ifndef GCDT_CELL_RANK_MAX
 #define GCDT_CELL_RANK_MAX 256
 #endif
@@ -882,10 +883,141 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort_block(KF kf, const int* en
     }
 }
 
-// eb[0..n): (element, bucket), grouped by bucket.  The reordered elements go
-// to out[0..n).
+// Buckets of more than CELL_BLOCK_MAX elements (a dense cluster: the count
+// passes flag them, so the common solve never takes this path) are sorted
+// together as (bucket, key, element) items: the items of every such bucket,
+// compacted in position order, are sorted in chunks of CELL_BLOCK_MAX in LDS
+// and then merged pairwise, each item placing itself in the merged run by a
+// binary search of the other run (keys are unique); item j of the sorted
+// array goes back to the j-th flagged position.
 template <class KF>
-static void cell_reorder(Ctx& c, const KF& kf, const int* end, const int2* eb, long long n, int* out)
+struct CellItem {
+    typename KF::Key key;
+    int bucket;
+    int elem;
+};
+
+template <class KF>
+__device__ inline bool item_less(const CellItem<KF>& x, const CellItem<KF>& y)
+{
+    return x.bucket < y.bucket || (x.bucket == y.bucket && KF::less(x.key, y.key));
+}
+
+__global__ __launch_bounds__(kBlock) void k_cell_huge_flags(const int* end, const int2* eb, long long n, int* flag)
+{
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (long long)gridDim.x * blockDim.x) {
+        const int b = eb[s].y;
+        flag[s] = end[b] - (b ? end[b - 1] : 0) > CELL_BLOCK_MAX ? 1 : 0;
+    }
+}
+
+template <class KF>
+__global__ __launch_bounds__(kBlock) void k_cell_huge_gather(KF kf, const int2* eb, long long n, const int* flag,
+                                                             const int* off, CellItem<KF>* items, int* pos)
+{
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (long long)gridDim.x * blockDim.x) {
+        if (!flag[s]) continue;
+        const int2 e = eb[s];
+        const int j = off[s];
+        items[j] = CellItem<KF>{kf.of(e.x), e.y, e.x};
+        pos[j] = (int)s;
+    }
+}
+
+// one workgroup per chunk of CELL_BLOCK_MAX items: bitonic sort in LDS, the
+// last chunk padded with items above every real one
+template <class KF>
+__global__ __launch_bounds__(kBlock) void k_cell_chunk_sort(CellItem<KF>* items, long long m)
+{
+    using Item = CellItem<KF>;
+    __shared__ Item sv[CELL_BLOCK_MAX];
+    const long long nchunks = (m + CELL_BLOCK_MAX - 1) / CELL_BLOCK_MAX;
+    for (long long q = blockIdx.x; q < nchunks; q += gridDim.x) {
+        const long long a = q * CELL_BLOCK_MAX;
+        const int k = (int)min((long long)CELL_BLOCK_MAX, m - a);
+        int np2 = 1;
+        while (np2 < k) np2 <<= 1;
+        for (int i = threadIdx.x; i < np2; i += kBlock)
+            sv[i] = i < k ? items[a + i] : Item{KF::max_key(), INT_MAX, -1};
+        for (int size = 2; size <= np2; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                __syncthreads();
+                for (int i = threadIdx.x; i < (np2 >> 1); i += kBlock) {
+                    const int lo = 2 * i - (i & (stride - 1)), hi = lo + stride;
+                    const bool up = (lo & size) == 0;
+                    if (item_less<KF>(sv[hi], sv[lo]) == up) {
+                        const Item t = sv[lo];
+                        sv[lo] = sv[hi];
+                        sv[hi] = t;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < k; i += kBlock) items[a + i] = sv[i];
+        __syncthreads();
+    }
+}
+
+// runs of w sorted items, merged in pairs: an item at index i of its run goes
+// to the pair's start + i + (items of the other run below it)
+template <class KF>
+__global__ __launch_bounds__(kBlock) void k_cell_merge(const CellItem<KF>* in, CellItem<KF>* out, long long m,
+                                                       long long w)
+{
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (long long)gridDim.x * blockDim.x) {
+        const long long run = j / w, base = (run & ~1LL) * w;
+        const long long o0 = (run ^ 1) * w, o1 = min(o0 + w, m);
+        const CellItem<KF> x = in[j];
+        long long lo = o0, hi = o1 > o0 ? o1 : o0;
+        while (lo < hi) {
+            const long long mid = (lo + hi) >> 1;
+            if (item_less<KF>(in[mid], x)) lo = mid + 1;
+            else hi = mid;
+        }
+        out[base + (j - run * w) + (lo - o0)] = x;
+    }
+}
+
+template <class KF>
+__global__ __launch_bounds__(kBlock) void k_cell_huge_put(const CellItem<KF>* items, const int* pos, long long m,
+                                                          int* out)
+{
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (long long)gridDim.x * blockDim.x)
+        out[pos[j]] = items[j].elem;
+}
+
+template <class KF>
+static void cell_reorder_huge(Ctx& c, const KF& kf, const int* end, const int2* eb, long long n, int* out)
+{
+    DBuf<int> flag((size_t)n), off((size_t)n + 1);  // the scan writes its total at off[n]
+    k_cell_huge_flags<<<grid_for(n), kBlock, 0, c.st>>>(end, eb, n, flag.p);
+    GCDT_LAUNCH_CHECK();
+    scan_excl_total(c, flag.p, off.p, n);
+    const long long m = c.get(C_TOTAL);
+    if (m <= 0) return;
+    DBuf<CellItem<KF>> a((size_t)m), b((size_t)m);
+    DBuf<int> pos((size_t)m);
+    k_cell_huge_gather<KF><<<grid_for(n), kBlock, 0, c.st>>>(kf, eb, n, flag.p, off.p, a.p, pos.p);
+    GCDT_LAUNCH_CHECK();
+    flag.release();
+    off.release();
+    k_cell_chunk_sort<KF><<<1024, kBlock, 0, c.st>>>(a.p, m);
+    GCDT_LAUNCH_CHECK();
+    CellItem<KF>*x = a.p, *y = b.p;
+    for (long long w = CELL_BLOCK_MAX; w < m; w <<= 1) {
+        k_cell_merge<KF><<<grid_for(m), kBlock, 0, c.st>>>(x, y, m, w);
+        GCDT_LAUNCH_CHECK();
+        std::swap(x, y);
+    }
+    k_cell_huge_put<KF><<<grid_for(m), kBlock, 0, c.st>>>(x, pos.p, m, out);
+    GCDT_LAUNCH_CHECK();
+}
+
+// eb[0..n): (element, bucket), grouped by bucket.  The reordered elements go
+// to out[0..n).  huge: some bucket holds more than CELL_BLOCK_MAX elements.
+template <class KF>
+static void cell_reorder(Ctx& c, const KF& kf, const int* end, const int2* eb, long long n, int* out, bool huge)
 {
     if (n <= 0) return;
     // a list entry per bucket above the rank cap: at most n / (cap + 1)
@@ -895,6 +1027,7 @@ static void cell_reorder(Ctx& c, const KF& kf, const int* end, const int2* eb, l
     GCDT_LAUNCH_CHECK();
     k_cell_sort_block<KF><<<1024, kBlock, 0, c.st>>>(kf, end, big.p, c.dcnt.p + C_NQ, eb, out);
     GCDT_LAUNCH_CHECK();
+    if (huge) cell_reorder_huge(c, kf, end, eb, n, out);
 }
 
 // Counting sort of points by 20-bit Morton cell: the count pass keeps the
@@ -902,10 +1035,13 @@ static void cell_reorder(Ctx& c, const KF& kf, const int* end, const int2* eb, l
 // inv[], the scatter places the point at the cell's start plus that rank,
 // each cell is reordered by input index, and the points and the inverse map
 // are written in that order.
-__global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n, int* cnt, int* rank)
+__global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n, int* cnt, int* rank, int* huge)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        rank[i] = atomicAdd(&cnt[morton_cell(pts[i])], 1);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int r = atomicAdd(&cnt[morton_cell(pts[i])], 1);
+        rank[i] = r;
+        if (r == CELL_BLOCK_MAX) atomicAdd(huge, 1);
+    }
 }
 
 // cell_end holds the inclusive scan of the counts: cell c starts at
@@ -933,14 +1069,15 @@ void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv
 {
     fill_i32(c, cell_end, 0, BITMAP_CELLS + 1);
     if (n <= 0) return;
-    k_morton_count<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv);
+    c.set(C_HUGE, 0);
+    k_morton_count<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv, c.dcnt.p + C_HUGE);
     GCDT_LAUNCH_CHECK();
     scan_incl(c, cell_end, cell_end, BITMAP_CELLS);
     {
         DBuf<int2> eb((size_t)n);
         k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv, eb.p);
         GCDT_LAUNCH_CHECK();
-        cell_reorder(c, CellKeyIndex{}, cell_end, eb.p, n, perm);
+        cell_reorder(c, CellKeyIndex{}, cell_end, eb.p, n, perm, c.get(C_HUGE) > 0);
     }
     k_morton_place<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, perm, out, inv);
     GCDT_LAUNCH_CHECK();
@@ -2189,10 +2326,13 @@ struct BatchCells {
 // points by (instance, fine cell), each cell in input order (the count,
 // scatter and per-cell reorder of the single solve's Morton order)
 __global__ __launch_bounds__(kBlock) void k_batch_cell_count(BatchCells b, const int2* pts, long long NP, int* cnt,
-                                                             int* rank)
+                                                             int* rank, int* huge)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x)
-        rank[i] = atomicAdd(&cnt[b.cell((int)i, pts[i])], 1);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
+        const int r = atomicAdd(&cnt[b.cell((int)i, pts[i])], 1);
+        rank[i] = r;
+        if (r == CELL_BLOCK_MAX) atomicAdd(huge, 1);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_cell_scatter(BatchCells b, const int2* pts, long long NP,
@@ -2211,14 +2351,15 @@ void batch_order(Ctx& c, const int2* pts, const int* vinst, const int* cbase, co
     fill_i32(c, cell_end, 0, ncells + 1);
     if (NP <= 0) return;
     BatchCells b{vinst, cbase, lvl};
-    k_batch_cell_count<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv);
+    c.set(C_HUGE, 0);
+    k_batch_cell_count<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv, c.dcnt.p + C_HUGE);
     GCDT_LAUNCH_CHECK();
     scan_incl(c, cell_end, cell_end, ncells);
     {
         DBuf<int2> eb((size_t)NP);
         k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv, eb.p);
         GCDT_LAUNCH_CHECK();
-        cell_reorder(c, CellKeyIndex{}, cell_end, eb.p, NP, perm);
+        cell_reorder(c, CellKeyIndex{}, cell_end, eb.p, NP, perm, c.get(C_HUGE) > 0);
     }
     k_morton_place<<<grid_for(NP), kBlock, 0, c.st>>>(pts, (int)NP, perm, out, inv);
     GCDT_LAUNCH_CHECK();
@@ -2384,7 +2525,7 @@ struct LeafKeyOne {
 // pass adds the cell's start to it without a second atomic.
 template <class KEY>
 __global__ __launch_bounds__(kBlock) void k_leaf_cells(KEY kf, const int2* pts, const int3* tri, const int* sons, int T,
-                                                       int* cnt, int* cellof, int* rank)
+                                                       int* cnt, int* cellof, int* rank, int* huge)
 {
     WAVE_LOOP(i, T)
     {
@@ -2400,6 +2541,7 @@ __global__ __launch_bounds__(kBlock) void k_leaf_cells(KEY kf, const int2* pts, 
         if (cell >= 0) {
             cellof[i] = cell;
             rank[i] = r;
+            if (r == CELL_BLOCK_MAX) atomicAdd(huge, 1);
         }
     }
 }
@@ -2467,10 +2609,14 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
 {
     DBuf<int> cnt((size_t)ncells + 1), cellof(T > 0 ? T : 1), dest(T > 0 ? T : 1);
     fill_i32(c, cnt.p, 0, ncells + 1);
-    k_leaf_cells<KEY><<<grid_for(T), kBlock, 0, c.st>>>(kf, pts, tri, sons, T, cnt.p, cellof.p, dest.p);
+    c.set(C_HUGE, 0);
+    k_leaf_cells<KEY><<<grid_for(T), kBlock, 0, c.st>>>(kf, pts, tri, sons, T, cnt.p, cellof.p, dest.p,
+                                                        c.dcnt.p + C_HUGE);
     GCDT_LAUNCH_CHECK();
     scan_excl_total(c, cnt.p, cnt.p, ncells);
-    int n = c.get(C_TOTAL);
+    c.read(C_TOTAL, C_HUGE - C_TOTAL + 1);
+    const int n = c.hcnt[C_TOTAL];
+    const bool huge = c.hcnt[C_HUGE] > 0;
     if (n > cap) throw Error(GCDT_ERR_CAPACITY, "more leaves than the output holds");
     DBuf<int> src(n > 0 ? n : 1);
     {
@@ -2480,7 +2626,7 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
         cellof.release();
         // each cell's leaves by their vertex ids (the count pass placed them
         // in arrival order); cnt holds the cells' starts, cnt[c + 1] their ends
-        cell_reorder(c, CellKeyTri{tri}, cnt.p + 1, eb.p, n, src.p);
+        cell_reorder(c, CellKeyTri{tri}, cnt.p + 1, eb.p, n, src.p, huge);
     }
     k_leaf_redest<<<grid_for(n), kBlock, 0, c.st>>>(src.p, n, dest.p);
     GCDT_LAUNCH_CHECK();

@@ -124,6 +124,7 @@ enum : int {
     C_NW2 = 23,       // insertion: triangles claimed for the next round
     C_STN_T = 24,     // crossing pass: triangles in use
     C_STN_S = 25,     // crossing pass: crossing vertices
+    C_HUGE = 26,      // count passes: cells of more elements than one workgroup sorts
     C_NUM = 32
 };
 

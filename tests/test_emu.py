@@ -164,12 +164,29 @@ def test_same_output_under_every_interleaving(case):
 
 def test_per_cell_reorder_paths_keep_valid_meshes():
     """With the per-cell reorder's caps at 3 and 8 elements, the lane-rank,
-    the workgroup and the keep-order paths all run on small meshes: every
-    result must still pass the validator (a reorder that lost or doubled an
-    element would corrupt the point or leaf order)."""
+    the workgroup and the large-cell merge paths all run on small meshes:
+    every result must still pass the validator (a reorder that lost or
+    doubled an element would corrupt the point or leaf order)."""
     lib = emu.build(emu.CELLCAP)
     for case in ({"kind": "lattice", "k": 64, "c": 8, "max_len": 12}, {"kind": "uniform", "n": 6000, "c": 40},
                  {"kind": "clustered", "n": 8000, "circles": 4},
                  {"kind": "batch", "sizes": [64, 700, 300], "seed": 9}):
         rc, out, err = emu.run_in_child(case, seed=5, conc=8, yield_atomics=1, lib=lib)
         assert rc == 0 and out is not None and out["valid"], (case, out, err)
+
+
+@pytest.mark.parametrize("case", [{"kind": "lattice", "k": 64, "c": 8, "max_len": 12},
+                                  {"kind": "clustered", "n": 6000, "circles": 4},
+                                  {"kind": "batch", "sizes": [64, 700, 300], "seed": 9}],
+                         ids=["lattice", "clustered", "batch"])
+def test_large_cells_are_deterministic_too(case):
+    """With the workgroup cap at 8 elements most cells take the large-cell
+    merge path; the arrays must still be the same under every interleaving
+    (cells above the cap once kept their arrival order)."""
+    lib = emu.build(emu.CELLCAP)
+    digests = set()
+    for seed, conc in ((1, 1), (2, 8), (3, 16)):
+        rc, out, err = emu.run_in_child(dict(case, digest=1), seed=seed, conc=conc, yield_atomics=1, lib=lib)
+        assert rc == 0 and out is not None and out["valid"], (case, out, err)
+        digests.add(out["digest"])
+    assert len(digests) == 1
