@@ -103,7 +103,7 @@ def test_noalloc_solve_equals_solve_on_leftovers():
     """gcdt_solve_noalloc (every buffer carved from the static scratch array)
     against gcdt_solve on three inputs in a row, the later ones smaller, so
     they run over the earlier calls' contents of the array."""
-    out = _run({"kind": "noalloc", "inputs": [["uniform", 30000, 300, 51], ["star", 8000, 52],
+    out = _run({"kind": "noalloc", "inputs": [["uniform", 12000, 120, 51], ["star", 5000, 52],
                                               ["lattice", 40, 20, 12]]},
                seed=4, conc=8, yield_atomics=1)
     assert out["status"] == [0, 0, 0], out
