@@ -18,14 +18,16 @@ __global__ __launch_bounds__(64) void k_steiner(Stn s, const int2* pieces, const
     }
 }
 
-// vertex -> an incident triangle
+// vertex -> its incident triangle of largest id (vt filled with -1 first):
+// the serial pass starts its walks there, and the triangles it splits and
+// rewrites follow from that start, so it is the same on every run
 __global__ __launch_bounds__(kBlock) void k_stn_vt(const int3* tri, int T, int* vt)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
         const int3 t = tri[i];
-        vt[t.x] = (int)i;
-        vt[t.y] = (int)i;
-        vt[t.z] = (int)i;
+        atomicMax(&vt[t.x], (int)i);
+        atomicMax(&vt[t.y], (int)i);
+        atomicMax(&vt[t.z], (int)i);
     }
 }
 

@@ -238,6 +238,9 @@ def run_case(case: dict) -> dict:
             res["status"] = e.status
             return res
         res["steiner"] = len(hp)
+        if case.get("digest"):
+            import hashlib
+            res["digest"] = hashlib.sha256(np.ascontiguousarray(tris).tobytes() + repr(hp).encode()).hexdigest()
         try:
             v = SC.check(g, c, tris, hp, brute=case.get("brute", False), augmented=case.get("augmented", False))
             res["unique"] = bool(v["unique"])

@@ -149,8 +149,9 @@ def test_steiner_capacity_is_reported():
 
 @pytest.mark.parametrize("case", [{"kind": "lattice", "k": 96, "c": 12, "max_len": 20},
                                   {"kind": "clustered", "n": 30000, "circles": 10},
-                                  {"kind": "batch", "sizes": [64, 300, 2000, 900, 150, 20000], "seed": 7}],
-                         ids=["lattice", "clustered", "batch"])
+                                  {"kind": "batch", "sizes": [64, 300, 2000, 900, 150, 20000], "seed": 7},
+                                  {"kind": "steiner", "gen": ["random", 9, 400, 40], "max_steiner": 2000}],
+                         ids=["lattice", "clustered", "batch", "crossings"])
 def test_same_output_under_every_interleaving(case):
     """Triangle and adjacency arrays are bit-identical whatever order the
     emulated workgroups and work-items run in (atomics arrive in a different

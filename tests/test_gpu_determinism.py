@@ -68,3 +68,15 @@ def test_cells_beyond_one_workgroup_are_deterministic():
         assert np.array_equal(O.canonical(t1), O.canonical(ref))
     else:
         assert O.validate(g, t1, c)["valid"]
+
+
+def test_crossing_constraints_are_deterministic():
+    """gcdt_solve_steiner: the same triangles, crossing vertices (numbered in
+    piece order) and named pieces on every run."""
+    from gcdt_amd.api import solve_steiner
+    from tests import steiner_cases as SC
+    g, c = SC.random_crossing(10, 20_000, 150)
+    t1, h1, s1, f1, _ = solve_steiner(g, c, 20_000)
+    t2, h2, s2, f2, _ = solve_steiner(g, c, 20_000)
+    assert len(h1) > 2000
+    assert np.array_equal(t1, t2) and h1 == h2 and np.array_equal(s1, s2) and np.array_equal(f1, f2)
