@@ -48,11 +48,11 @@ def _checker(args):
 #   ins_claim : loc 4 + point id 4 + 1.x claim atomics 8 (first round of a run only)
 #   ins_split : per claimed triangle: list 4 + claim 8 + winner loc 4 + key 8 + tri/adj 24
 #               + 3 children x (tri 12 + adj 12) written + sons 4 + loc 4
-#   flip_apply: per proposal meta 4 + own claim 8; a winning proposal (92.8% of them on
+#   flip_apply: per proposal meta 4 + own claim 8; a winning proposal (92.2% of them on
 #               the 10M workload) adds its own vertices, neighbours and coordinates 48,
 #               the neighbour record's 60, both records rewritten 2 x 56 and 4 list
 #               entries 16; a losing one the winner's claim 8, its mark 4 and a list
-#               entry 4: 12 + 0.928 x 236 + 0.072 x 16 = 232
+#               entry 4: 12 + 0.922 x 236 + 0.078 x 16 = 231
 #   locate_bitmap: point 8 + isv 4 + 2 offsets 8 + ~1.5 x (tri id 4 + tri 12 + 3 points 24) + loc 4 + list 4
 # With the packed records, each flip_test launch also settles the previous
 # round's flips (listed as flip_test_fix items): per settled triangle, 2 x
@@ -63,7 +63,7 @@ BYTES_PER_ITEM = {
     "flip_test": 132,
     "ins_reloc": 84,
     "ins_claim": 16,
-    "flip_apply": 232,
+    "flip_apply": 231,
     "locate_bitmap": 88,
     "ins_split": 128,
 }
