@@ -786,7 +786,7 @@ struct CellKeyTri {
 // up to CELL_RANK_MAX at its place in the bucket's key order and every other
 // element where it is; a bucket's first lane lists a bucket of
 // CELL_RANK_MAX + 1 .. CELL_BLOCK_MAX for the workgroup pass.  The workgroup
-// first stages the keys of every position its rankable buckets cover (its own
+// first stages the keys of every position its ranked buckets cover (its own
 // kBlock positions and the parts of its edge buckets beyond them, at most
 // CELL_RANK_MAX - 1 on either side), so the rank loops read only LDS.
 template <class KF>
@@ -819,7 +819,12 @@ __global__ __launch_bounds__(kBlock) void k_cell_rank(KF kf, const int* end, con
         }
         __syncthreads();
         const int lo = s_lo, hi = s_hi;
-        for (int t = lo + (int)threadIdx.x; t < hi; t += kBlock) sk[t - lo] = kf.of(eb[s0 + t].x);
+        // a lane stages its own key when its cell is ranked (the keys of
+        // unranked positions are never read); the parts of the edge cells
+        // beyond the workgroup's positions are staged in turn
+        if (rank) sk[(int)threadIdx.x - lo] = kf.of(e.x);
+        for (int t = lo + (int)threadIdx.x; t < 0; t += kBlock) sk[t - lo] = kf.of(eb[s0 + t].x);
+        for (int t = kBlock + (int)threadIdx.x; t < hi; t += kBlock) sk[t - lo] = kf.of(eb[s0 + t].x);
         __syncthreads();
         bool list = false;
         if (s < n) {
