@@ -813,9 +813,17 @@ __global__ __launch_bounds__(kBlock) void k_cell_rank(KF kf, const int* end, con
         __syncthreads();
         // window [lo, hi) relative to s0: lo >= -(CELL_RANK_MAX - 1),
         // hi <= kBlock + CELL_RANK_MAX - 1
-        if (rank) {
-            atomicMin(&s_lo, (int)(a - s0));
-            atomicMax(&s_hi, (int)(a + k - s0));
+        {
+            // one LDS atomic per wave: the wave's own bounds first
+            int wl = rank ? (int)(a - s0) : kBlock, wh = rank ? (int)(a + k - s0) : 0;
+            for (int d = 32; d > 0; d >>= 1) {
+                wl = min(wl, __shfl_down(wl, d, 64));
+                wh = max(wh, __shfl_down(wh, d, 64));
+            }
+            if (__lane_id() == 0 && wl < wh) {
+                atomicMin(&s_lo, wl);
+                atomicMax(&s_hi, wh);
+            }
         }
         __syncthreads();
         const int lo = s_lo, hi = s_hi;
