@@ -1532,11 +1532,12 @@ __global__ __launch_bounds__(TAIL_BS) void k_ins_tail(Ins s, int* P, int* P2, in
             s_tb = s_nt;
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += TAIL_BS) {
-            int p = P[i];
-            int need = ins_need(s, p, ep);
+        for (int i0 = 0; i0 < n; i0 += TAIL_BS) {
+            const int i = i0 + (int)threadIdx.x;
+            const int p = i < n ? P[i] : -1;
+            const int need = p >= 0 ? ins_need(s, p, ep) : 0;
+            const int b = wave_alloc(need, &s_nt);
             if (need) {
-                int b = atomicAdd(&s_nt, need);
                 if (b + need > s.cap) atomicOr(s.err, MERR_CAPACITY);
                 else ins_split(s, p, b);
             }
@@ -1545,9 +1546,12 @@ __global__ __launch_bounds__(TAIL_BS) void k_ins_tail(Ins s, int* P, int* P2, in
         int tb = s_tb, te = min(s_nt, s.cap);
         for (int c = tb + (int)threadIdx.x; c < te; c += TAIL_BS) ins_fix(s, c);
         __syncthreads();
-        for (int i = threadIdx.x; i < n; i += TAIL_BS) {
-            int p = P[i];
-            if (ins_relocate(s, p)) P2[atomicAdd(&s_np, 1)] = p;
+        for (int i0 = 0; i0 < n; i0 += TAIL_BS) {
+            const int i = i0 + (int)threadIdx.x;
+            const int p = i < n ? P[i] : -1;
+            const bool keep = p >= 0 && ins_relocate(s, p);
+            const int b = wave_alloc(keep ? 1 : 0, &s_np);
+            if (keep) P2[b] = p;
         }
         __syncthreads();
         int n2 = s_np;
@@ -1935,20 +1939,25 @@ __global__ __launch_bounds__(TAIL_BS) void k_flip_tail(E e, int* A, int* B, int*
             s_tag[q ^ 1][i] = -1;
             s_key[q ^ 1][i] = 0ull;
         }
-        for (int i = threadIdx.x; i < n; i += TAIL_BS) {
-            int t = A[i], nn = -1;
-            int res = lds ? e.apply_l(t, ep, &nn, cl) : e.template apply<true>(t, ep, &nn);
+        // block-uniform trip count, so each wave reserves its appends with
+        // one LDS atomic per counter
+        for (int i0 = 0; i0 < n; i0 += TAIL_BS) {
+            const int i = i0 + (int)threadIdx.x;
+            int t = -1, nn = -1, res = 0;
+            if (i < n) {
+                t = A[i];
+                res = lds ? e.apply_l(t, ep, &nn, cl) : e.template apply<true>(t, ep, &nn);
+            }
+            const int b = wave_alloc(res == 1 ? 1 : res == 2 ? 2 : 0, &s_n2[q]);
+            const int w = wave_alloc(res == 2 ? 2 : 0, &s_nf[q]);
             if (res == 1) {
-                int b = atomicAdd(&s_n2[q], 1);
                 if (b < cap) B[b] = t;
                 e.set_mark(t, (int)ep + 1);
             } else if (res == 2) {
-                int b = atomicAdd(&s_n2[q], 2);
                 if (b + 1 < cap) {
                     B[b] = t;
                     B[b + 1] = nn;
                 }
-                int w = atomicAdd(&s_nf[q], 2);
                 if (w + 1 < cap) {
                     lf[w] = t;
                     lf[w + 1] = nn;
