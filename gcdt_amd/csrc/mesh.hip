@@ -754,20 +754,44 @@ constexpr int CELL_RANK_MAX = GCDT_CELL_RANK_MAX;
 constexpr int CELL_BLOCK_MAX = GCDT_CELL_BLOCK_MAX;
 static_assert(CELL_RANK_MAX <= CELL_BLOCK_MAX, "bucket caps");
 
-// keys: a point's input index (the element itself)
-struct CellKeyIndex {
+// Points: the key is the input index; a position holds (index, cell, x, y),
+// so placing a point writes its coordinates, its index and the inverse map
+// without gathering the point again.
+struct CellKeyPoint {
     struct Key {
         int a;
     };
+    using Slot = int4;
+    const int2* pts;  // input points (for the paths that place by element)
+    int* perm;        // sorted position -> input index
+    int2* out;        // sorted points
+    int* inv;         // input index -> sorted position
     __device__ Key of(int e) const { return Key{e}; }
     __device__ static Key max_key() { return Key{INT_MAX}; }
     __device__ static bool less(const Key& x, const Key& y) { return x.a < y.a; }
+    __device__ void put(long long pos, const Slot& e) const
+    {
+        perm[pos] = e.x;
+        out[pos] = make_int2(e.z, e.w);
+        inv[e.x] = (int)pos;
+    }
+    __device__ void put_elem(long long pos, int el) const
+    {
+        perm[pos] = el;
+        out[pos] = pts[el];
+        inv[el] = (int)pos;
+    }
 };
 
-// keys: a triangle's vertex ids, smallest first
+// Leaves: the key is the triangle's vertex ids, smallest first; a position
+// holds (triangle, cell) and placing writes the triangle id.
 struct CellKeyTri {
     const int3* tri;
+    int* src;  // sorted position -> triangle
     using Key = int3;
+    using Slot = int2;
+    __device__ void put(long long pos, const Slot& e) const { src[pos] = e.x; }
+    __device__ void put_elem(long long pos, int el) const { src[pos] = el; }
     __device__ Key of(int e) const
     {
         const int3 t = tri[e];
@@ -790,15 +814,16 @@ struct CellKeyTri {
 // kBlock positions and the parts of its edge buckets beyond them, at most
 // CELL_RANK_MAX - 1 on either side), so the rank loops read only LDS.
 template <class KF>
-__global__ __launch_bounds__(kBlock) void k_cell_rank(KF kf, const int* end, const int2* eb, long long n, int* out,
+__global__ __launch_bounds__(kBlock) void k_cell_rank(KF kf, const int* end, const typename KF::Slot* eb, long long n,
                                                      int* big, int* nbig)
 {
     using Key = typename KF::Key;
+    using Slot = typename KF::Slot;
     __shared__ Key sk[kBlock + 2 * (CELL_RANK_MAX - 1)];
     __shared__ int s_lo, s_hi;
     for (long long s0 = (long long)blockIdx.x * kBlock; s0 < n; s0 += (long long)gridDim.x * kBlock) {
         const long long s = s0 + threadIdx.x;
-        int2 e = make_int2(0, 0);
+        Slot e{};
         long long a = 0, k = 0;
         if (s < n) {
             e = eb[s];
@@ -844,7 +869,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_rank(KF kf, const int* end, con
                 for (int q = 0; q < (int)k; q++) r += KF::less(w[q], mine) ? 1 : 0;
                 pos = a + r;
             }
-            out[pos] = e.x;
+            kf.put(pos, e);
             list = s == a && k > CELL_RANK_MAX && k <= CELL_BLOCK_MAX;
         }
         const int p = wave_alloc(list ? 1 : 0, nbig);
@@ -857,7 +882,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_rank(KF kf, const int* end, con
 // two with largest keys, of the bucket's elements into its range of out[]
 template <class KF>
 __global__ __launch_bounds__(kBlock) void k_cell_sort_block(KF kf, const int* end, const int* big, const int* nbig,
-                                                              const int2* eb, int* out)
+                                                              const typename KF::Slot* eb)
 {
     using Key = typename KF::Key;
     __shared__ int se[CELL_BLOCK_MAX];
@@ -891,7 +916,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_sort_block(KF kf, const int* en
             }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < k; i += kBlock) out[a + i] = se[i];
+        for (int i = threadIdx.x; i < k; i += kBlock) kf.put_elem(a + i, se[i]);
         __syncthreads();
     }
 }
@@ -916,7 +941,8 @@ __device__ inline bool item_less(const CellItem<KF>& x, const CellItem<KF>& y)
     return x.bucket < y.bucket || (x.bucket == y.bucket && KF::less(x.key, y.key));
 }
 
-__global__ __launch_bounds__(kBlock) void k_cell_huge_flags(const int* end, const int2* eb, long long n, int* flag)
+template <class SLOT>
+__global__ __launch_bounds__(kBlock) void k_cell_huge_flags(const int* end, const SLOT* eb, long long n, int* flag)
 {
     for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (long long)gridDim.x * blockDim.x) {
         const int b = eb[s].y;
@@ -925,12 +951,13 @@ __global__ __launch_bounds__(kBlock) void k_cell_huge_flags(const int* end, cons
 }
 
 template <class KF>
-__global__ __launch_bounds__(kBlock) void k_cell_huge_gather(KF kf, const int2* eb, long long n, const int* flag,
+__global__ __launch_bounds__(kBlock) void k_cell_huge_gather(KF kf, const typename KF::Slot* eb, long long n,
+                                                             const int* flag,
                                                              const int* off, CellItem<KF>* items, int* pos)
 {
     for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (long long)gridDim.x * blockDim.x) {
         if (!flag[s]) continue;
-        const int2 e = eb[s];
+        const typename KF::Slot e = eb[s];
         const int j = off[s];
         items[j] = CellItem<KF>{kf.of(e.x), e.y, e.x};
         pos[j] = (int)s;
@@ -993,18 +1020,17 @@ __global__ __launch_bounds__(kBlock) void k_cell_merge(const CellItem<KF>* in, C
 }
 
 template <class KF>
-__global__ __launch_bounds__(kBlock) void k_cell_huge_put(const CellItem<KF>* items, const int* pos, long long m,
-                                                          int* out)
+__global__ __launch_bounds__(kBlock) void k_cell_huge_put(KF kf, const CellItem<KF>* items, const int* pos, long long m)
 {
     for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (long long)gridDim.x * blockDim.x)
-        out[pos[j]] = items[j].elem;
+        kf.put_elem(pos[j], items[j].elem);
 }
 
 template <class KF>
-static void cell_reorder_huge(Ctx& c, const KF& kf, const int* end, const int2* eb, long long n, int* out)
+static void cell_reorder_huge(Ctx& c, const KF& kf, const int* end, const typename KF::Slot* eb, long long n)
 {
     DBuf<int> flag((size_t)n), off((size_t)n + 1);  // the scan writes its total at off[n]
-    k_cell_huge_flags<<<grid_for(n), kBlock, 0, c.st>>>(end, eb, n, flag.p);
+    k_cell_huge_flags<typename KF::Slot><<<grid_for(n), kBlock, 0, c.st>>>(end, eb, n, flag.p);
     GCDT_LAUNCH_CHECK();
     scan_excl_total(c, flag.p, off.p, n);
     const long long m = c.get(C_TOTAL);
@@ -1023,31 +1049,32 @@ static void cell_reorder_huge(Ctx& c, const KF& kf, const int* end, const int2* 
         GCDT_LAUNCH_CHECK();
         std::swap(x, y);
     }
-    k_cell_huge_put<KF><<<grid_for(m), kBlock, 0, c.st>>>(x, pos.p, m, out);
+    k_cell_huge_put<KF><<<grid_for(m), kBlock, 0, c.st>>>(kf, x, pos.p, m);
     GCDT_LAUNCH_CHECK();
 }
 
-// eb[0..n): (element, bucket), grouped by bucket.  The reordered elements go
-// to out[0..n).  huge: some bucket holds more than CELL_BLOCK_MAX elements.
+// eb[0..n): (element, bucket, ...), grouped by bucket.  Each element is
+// placed (KF::put) at its position in the reordered order.  huge: some bucket
+// holds more than CELL_BLOCK_MAX elements.
 template <class KF>
-static void cell_reorder(Ctx& c, const KF& kf, const int* end, const int2* eb, long long n, int* out, bool huge)
+static void cell_reorder(Ctx& c, const KF& kf, const int* end, const typename KF::Slot* eb, long long n, bool huge)
 {
     if (n <= 0) return;
     // a list entry per bucket above the rank cap: at most n / (cap + 1)
     DBuf<int> big((size_t)(n / (CELL_RANK_MAX + 1) + 1));
     c.set(C_NQ, 0);
-    k_cell_rank<KF><<<grid_for(n), kBlock, 0, c.st>>>(kf, end, eb, n, out, big.p, c.dcnt.p + C_NQ);
+    k_cell_rank<KF><<<grid_for(n), kBlock, 0, c.st>>>(kf, end, eb, n, big.p, c.dcnt.p + C_NQ);
     GCDT_LAUNCH_CHECK();
-    k_cell_sort_block<KF><<<1024, kBlock, 0, c.st>>>(kf, end, big.p, c.dcnt.p + C_NQ, eb, out);
+    k_cell_sort_block<KF><<<1024, kBlock, 0, c.st>>>(kf, end, big.p, c.dcnt.p + C_NQ, eb);
     GCDT_LAUNCH_CHECK();
-    if (huge) cell_reorder_huge(c, kf, end, eb, n, out);
+    if (huge) cell_reorder_huge(c, kf, end, eb, n);
 }
 
 // Counting sort of points by 20-bit Morton cell: the count pass keeps the
 // value its atomicAdd returns (the point's arrival rank in its cell) in
-// inv[], the scatter places the point at the cell's start plus that rank,
-// each cell is reordered by input index, and the points and the inverse map
-// are written in that order.
+// inv[], the scatter writes (index, cell, x, y) at the cell's start plus that
+// rank, and the per-cell reorder places each point by input index, writing
+// the sorted points, perm and the inverse map.
 __global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n, int* cnt, int* rank, int* huge)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -1060,21 +1087,13 @@ __global__ __launch_bounds__(kBlock) void k_morton_count(const int2* pts, int n,
 // cell_end holds the inclusive scan of the counts: cell c starts at
 // cell_end[c - 1] (0 for c = 0).
 __global__ __launch_bounds__(kBlock) void k_morton_scatter(const int2* pts, int n, const int* cell_end,
-                                                           const int* rank, int2* eb)
+                                                           const int* rank, int4* eb)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const int cell = (int)morton_cell(pts[i]);
+        const int2 p = pts[i];
+        const int cell = (int)morton_cell(p);
         const int pos = (cell ? cell_end[cell - 1] : 0) + rank[i];
-        eb[pos] = make_int2((int)i, cell);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_morton_place(const int2* pts, int n, const int* perm, int2* out, int* inv)
-{
-    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < n; s += (long long)gridDim.x * blockDim.x) {
-        const int i = perm[s];
-        out[s] = pts[i];
-        inv[i] = (int)s;
+        eb[pos] = make_int4((int)i, cell, p.x, p.y);
     }
 }
 
@@ -1086,14 +1105,11 @@ void morton_order(Ctx& c, const int2* pts, int n, int2* out, int* perm, int* inv
     k_morton_count<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv, c.dcnt.p + C_HUGE);
     GCDT_LAUNCH_CHECK();
     scan_incl(c, cell_end, cell_end, BITMAP_CELLS);
-    {
-        DBuf<int2> eb((size_t)n);
-        k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv, eb.p);
-        GCDT_LAUNCH_CHECK();
-        cell_reorder(c, CellKeyIndex{}, cell_end, eb.p, n, perm, c.get(C_HUGE) > 0);
-    }
-    k_morton_place<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, perm, out, inv);
+    DBuf<int4> eb((size_t)n);
+    k_morton_scatter<<<grid_for(n), kBlock, 0, c.st>>>(pts, n, cell_end, inv, eb.p);
     GCDT_LAUNCH_CHECK();
+    // inv held the arrival ranks; placing rewrites it as the inverse map
+    cell_reorder(c, CellKeyPoint{pts, perm, out, inv}, cell_end, eb.p, n, c.get(C_HUGE) > 0);
 }
 
 // Sample of Morton-ordered points: the 256x256 sample cell with Morton index
@@ -2358,12 +2374,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_cell_count(BatchCells b, const
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_cell_scatter(BatchCells b, const int2* pts, long long NP,
-                                                               const int* cell_end, const int* rank, int2* eb)
+                                                               const int* cell_end, const int* rank, int4* eb)
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < NP; i += (long long)gridDim.x * blockDim.x) {
-        const int cell = b.cell((int)i, pts[i]);
+        const int2 p = pts[i];
+        const int cell = b.cell((int)i, p);
         const int pos = (cell ? cell_end[cell - 1] : 0) + rank[i];
-        eb[pos] = make_int2((int)i, cell);
+        eb[pos] = make_int4((int)i, cell, p.x, p.y);
     }
 }
 
@@ -2377,14 +2394,10 @@ void batch_order(Ctx& c, const int2* pts, const int* vinst, const int* cbase, co
     k_batch_cell_count<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv, c.dcnt.p + C_HUGE);
     GCDT_LAUNCH_CHECK();
     scan_incl(c, cell_end, cell_end, ncells);
-    {
-        DBuf<int2> eb((size_t)NP);
-        k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv, eb.p);
-        GCDT_LAUNCH_CHECK();
-        cell_reorder(c, CellKeyIndex{}, cell_end, eb.p, NP, perm, c.get(C_HUGE) > 0);
-    }
-    k_morton_place<<<grid_for(NP), kBlock, 0, c.st>>>(pts, (int)NP, perm, out, inv);
+    DBuf<int4> eb((size_t)NP);
+    k_batch_cell_scatter<<<grid_for(NP), kBlock, 0, c.st>>>(b, pts, NP, cell_end, inv, eb.p);
     GCDT_LAUNCH_CHECK();
+    cell_reorder(c, CellKeyPoint{pts, perm, out, inv}, cell_end, eb.p, NP, c.get(C_HUGE) > 0);
 }
 
 // Sorted range of 16x16 cell c (Morton index) of instance i: the fine cells
@@ -2648,7 +2661,7 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
         cellof.release();
         // each cell's leaves by their vertex ids (the count pass placed them
         // in arrival order); cnt holds the cells' starts, cnt[c + 1] their ends
-        cell_reorder(c, CellKeyTri{tri}, cnt.p + 1, eb.p, n, src.p, huge);
+        cell_reorder(c, CellKeyTri{tri, src.p}, cnt.p + 1, eb.p, n, huge);
     }
     k_leaf_redest<<<grid_for(n), kBlock, 0, c.st>>>(src.p, n, dest.p);
     GCDT_LAUNCH_CHECK();
