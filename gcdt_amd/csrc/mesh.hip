@@ -784,14 +784,20 @@ struct CellKeyPoint {
 };
 
 // Leaves: the key is the triangle's vertex ids, smallest first; a position
-// holds (triangle, cell) and placing writes the triangle id.
+// holds (triangle, cell) and placing writes the triangle id and its
+// destination.
 struct CellKeyTri {
     const int3* tri;
-    int* src;  // sorted position -> triangle
+    int* src;   // sorted position -> triangle
+    int* dest;  // triangle -> sorted position
     using Key = int3;
     using Slot = int2;
-    __device__ void put(long long pos, const Slot& e) const { src[pos] = e.x; }
-    __device__ void put_elem(long long pos, int el) const { src[pos] = el; }
+    __device__ void put(long long pos, const Slot& e) const { put_elem(pos, e.x); }
+    __device__ void put_elem(long long pos, int el) const
+    {
+        src[pos] = el;
+        dest[el] = (int)pos;
+    }
     __device__ Key of(int e) const
     {
         const int3 t = tri[e];
@@ -2581,8 +2587,9 @@ __global__ __launch_bounds__(kBlock) void k_leaf_cells(KEY kf, const int2* pts, 
     }
 }
 
-// dest[] holds the ranks on entry and the destinations on exit; src[] gets
-// the inverse (the leaf that lands at each destination).
+// dest[] holds the ranks on entry and the arrival-order destinations on exit
+// (-1 for dead triangles); eb[] gets (leaf, cell) at each destination.  The
+// per-cell reorder then writes the final destinations and their inverse.
 __global__ __launch_bounds__(kBlock) void k_leaf_dest(const int* sons, const int* cellof, int T, const int* start,
                                                       int* dest, int2* eb, long long cap)
 {
@@ -2591,13 +2598,6 @@ __global__ __launch_bounds__(kBlock) void k_leaf_dest(const int* sons, const int
         dest[i] = d;
         if (d >= 0 && d < cap) eb[d] = make_int2((int)i, cellof[i]);
     }
-}
-
-// dest[] from the reordered src[]
-__global__ __launch_bounds__(kBlock) void k_leaf_redest(const int* src, int n, int* dest)
-{
-    for (long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (long long)gridDim.x * blockDim.x)
-        dest[src[o]] = (int)o;
 }
 
 // Where the moved leaves go: plain tri/adj arrays, or straight into the
@@ -2661,10 +2661,8 @@ static int filter_leaves_sorted(Ctx& c, const KEY& kf, long long ncells, const i
         cellof.release();
         // each cell's leaves by their vertex ids (the count pass placed them
         // in arrival order); cnt holds the cells' starts, cnt[c + 1] their ends
-        cell_reorder(c, CellKeyTri{tri, src.p}, cnt.p + 1, eb.p, n, huge);
+        cell_reorder(c, CellKeyTri{tri, src.p, dest.p}, cnt.p + 1, eb.p, n, huge);
     }
-    k_leaf_redest<<<grid_for(n), kBlock, 0, c.st>>>(src.p, n, dest.p);
-    GCDT_LAUNCH_CHECK();
     if (n > 0) {
         ProfScope ps("leaf_move", c.st, n);
         k_leaf_move<OUT><<<grid_for(n), kBlock, 0, c.st>>>(tri, adj, dest.p, src.p, n, out);
