@@ -738,8 +738,9 @@ __device__ inline unsigned morton_cell(int2 p)
 // claims, triangle ids key the flip claims), each bucket's range is then
 // reordered by a key of its own elements -- the input index of a point, the
 // sorted vertex ids of a triangle -- so the order is the same on every run.
-// The scatter writes one int2 (element, bucket) per position.  A bucket of up
-// to CELL_RANK_MAX (256) elements is ranked by one lane per element, its key
+// The scatter writes one slot per position: (element, bucket) for triangles,
+// (element, bucket, x, y) for points.  A bucket of up to CELL_RANK_MAX (256)
+// elements is ranked by one lane per element, its key
 // against every key of the bucket (all staged in LDS by the workgroup); buckets
 // of up to CELL_BLOCK_MAX by one workgroup's bitonic sort in LDS; larger ones,
 // which the count passes flag, by a merge sort of all their elements
@@ -811,10 +812,10 @@ struct CellKeyTri {
     }
 };
 
-// eb[s] = (element, bucket) at position s; a bucket's range is
-// [end[b-1], end[b]) (end: inclusive scan).  Writes each element of a bucket of
-// up to CELL_RANK_MAX at its place in the bucket's key order and every other
-// element where it is; a bucket's first lane lists a bucket of
+// eb[s] = (element, bucket, ...) at position s; a bucket's range is
+// [end[b-1], end[b]) (end: inclusive scan).  Places (KF::put) each element of a
+// bucket of up to CELL_RANK_MAX at its rank in the bucket's key order and
+// every other element where it is; a bucket's first lane lists a bucket of
 // CELL_RANK_MAX + 1 .. CELL_BLOCK_MAX for the workgroup pass.  The workgroup
 // first stages the keys of every position its ranked buckets cover (its own
 // kBlock positions and the parts of its edge buckets beyond them, at most
