@@ -140,16 +140,85 @@ __global__ __launch_bounds__(kBlock) void k_cav_check(Cons c, const int2* pieces
     }
 }
 
-// R3: collect the cavity and retriangulate it in place.
+// Cavities of more crossed triangles than this are retriangulated by a whole
+// wave (k_cav_retri_wave): a long piece's pseudo-polygons have hundreds of
+// vertices, and one thread's Delaunay-vertex scans over them took most of a
+// solve with long segments.
+#ifndef GCDT_CAV_WAVE_K
+#define GCDT_CAV_WAVE_K 64
+#endif
+constexpr int CAV_WAVE_K = GCDT_CAV_WAVE_K;
+
+// R3: collect the cavity and retriangulate it in place (one thread per cavity
+// of up to CAV_WAVE_K triangles).
 __global__ __launch_bounds__(kBlock) void k_cav_retri(Cons c, const int2* pieces, const Win* wins, const int* nwin,
                                                       int* scr, unsigned epoch, int* rew, int* nrew)
 {
     int nw = *nwin;
     for (long long wi = (long long)blockIdx.x * blockDim.x + threadIdx.x; wi < nw; wi += (long long)gridDim.x * blockDim.x) {
-        if (!cav_retri_one(c, pieces, wins, (int)wi, scr, epoch)) continue;
         Win w = wins[wi];
+        if (w.K > CAV_WAVE_K) continue;
+        if (!cav_retri_one(c, pieces, wins, (int)wi, scr, epoch)) continue;
         int base = atomicAdd(nrew, w.K);
         for (int j = 0; j < w.K; j++) rew[base + j] = scr[w.off + j];
+    }
+}
+
+// Delaunay vertex found by the 64 lanes of a wave: each scans every 64th
+// candidate as DvScan does, then a butterfly keeps the better of two
+// candidates -- the one strictly inside the other's circle, the lower index
+// when cocircular -- which is the vertex DvScan returns.
+struct DvWave {
+    __device__ static int find(const int* PX, const int* PY, int lo, int hi, int2 PA, int2 PB)
+    {
+        const int lane = __lane_id();
+        int ci = -1;
+        int2 PC = make_int2(0, 0);
+        for (int i = lo + lane; i < hi; i += 64) {
+            const int2 PI = make_int2(PX[i], PY[i]);
+            if (ci < 0 || incircle(PA, PB, PC, PI) > 0) {
+                ci = i;
+                PC = PI;
+            }
+        }
+        for (int d = 32; d > 0; d >>= 1) {
+            const int oi = __shfl(ci, lane ^ d, 64);
+            const int ox = __shfl(PC.x, lane ^ d, 64), oy = __shfl(PC.y, lane ^ d, 64);
+            if (oi < 0) continue;
+            const int2 PO = make_int2(ox, oy);
+            bool take = ci < 0;
+            if (!take) {
+                const int r = incircle(PA, PB, PC, PO);
+                take = r > 0 || (r == 0 && oi < ci);
+            }
+            if (take) {
+                ci = oi;
+                PC = PO;
+            }
+        }
+        return ci;
+    }
+};
+
+// R3 for cavities above CAV_WAVE_K: every lane of the wave runs the body for
+// the same cavity (the same loads and identical stores, so each lane reads
+// its own writes) and the lanes share the Delaunay-vertex scans.
+__global__ __launch_bounds__(kBlock) void k_cav_retri_wave(Cons c, const int2* pieces, const Win* wins,
+                                                           const int* nwin, int* scr, unsigned epoch, int* rew,
+                                                           int* nrew)
+{
+    const int nw = *nwin;
+    const int lane = __lane_id();
+    const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long wi = wave0; wi < nw; wi += nwaves) {
+        const Win w = wins[wi];
+        if (w.K <= CAV_WAVE_K) continue;
+        if (!cav_retri_one<DvWave>(c, pieces, wins, (int)wi, scr, epoch)) continue;
+        int base = 0;
+        if (lane == 0) base = atomicAdd(nrew, w.K);
+        base = __shfl(base, 0, 64);
+        for (int j = lane; j < w.K; j += 64) rew[base + j] = scr[w.off + j];
     }
 }
 
@@ -589,7 +658,8 @@ ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, int nmax, c
         k_cav_claim<<<g, kBlock, 0, cx.st>>>(c, pieces.p, Q, cx.dcnt.p + C_NQ, ep, ncross.p, cx.dcnt.p + C_TOTAL);
         GCDT_LAUNCH_CHECK();
         long long tot = cx.get(C_TOTAL);
-        long long cap = 11LL * tot + 16LL * nq + 1024;
+        // sum of cav_scratch_ints(K) over the queued pieces (13 K + 20 each)
+        long long cap = 13LL * tot + 20LL * nq + 1024;
         if (cap > (1LL << 30)) cap = 1LL << 30;
         scr.ensure((size_t)cap);
         rew.ensure((size_t)tot + 1);
@@ -598,6 +668,11 @@ ConsResult insert_constraints(Ctx& cx, const Mesh& m, int T, int nv, int nmax, c
         GCDT_LAUNCH_CHECK();
         k_cav_retri<<<g, kBlock, 0, cx.st>>>(c, pieces.p, wins.p, cx.dcnt.p + C_NWIN, scr.p, ep, rew.p,
                                              cx.dcnt.p + C_NREW);
+        GCDT_LAUNCH_CHECK();
+        // one wave per winner, up to 8192 waves in flight
+        const int gw = (int)((std::min<long long>(nq, 8192) * 64 + kBlock - 1) / kBlock);
+        k_cav_retri_wave<<<gw, kBlock, 0, cx.st>>>(c, pieces.p, wins.p, cx.dcnt.p + C_NWIN, scr.p, ep, rew.p,
+                                                   cx.dcnt.p + C_NREW);
         GCDT_LAUNCH_CHECK();
         k_cav_fix<<<grid_for(tot), kBlock, 0, cx.st>>>(c, wins.p, scr.p, rew.p, cx.dcnt.p + C_NREW, ep);
         GCDT_LAUNCH_CHECK();

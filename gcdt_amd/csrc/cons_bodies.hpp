@@ -192,7 +192,7 @@ __host__ __device__ inline int cav_walk(const Cons& c, int a, int b, bool mark, 
 }
 
 // R2: winners reserve scratch, losers stay queued
-__host__ __device__ inline long long cav_scratch_ints(int K) { return 11LL * K + 16; }
+__host__ __device__ inline long long cav_scratch_ints(int K) { return 13LL * K + 20; }
 
 struct Win {
     int piece;
@@ -201,8 +201,32 @@ struct Win {
     int off;  // scratch offset (ints)
 };
 
+// The Delaunay vertex of base (A, B) among the polygon vertices lo..hi-1
+// (coordinates in PX/PY): the one whose circle with A and B holds no other
+// strictly inside, the first in order among cocircular ones.  One thread
+// scans them in order, replacing its candidate by any vertex strictly inside
+// the candidate's circle.
+struct DvScan {
+    __host__ __device__ static int find(const int* PX, const int* PY, int lo, int hi, int2 PA, int2 PB)
+    {
+        int ci = lo;
+        int2 PC = make_int2(PX[lo], PY[lo]);
+        for (int i = lo + 1; i < hi; i++) {
+            const int2 PI = make_int2(PX[i], PY[i]);
+            if (incircle(PA, PB, PC, PI) > 0) {
+                ci = i;
+                PC = PI;
+            }
+        }
+        return ci;
+    }
+};
+
 // R3 body: collect winner wi's cavity and retriangulate it in place.
-// Returns false (and flags MERR_CAVITY) on an inconsistent cavity.
+// Returns false (and flags MERR_CAVITY) on an inconsistent cavity.  DV finds
+// each Delaunay vertex (DvScan, or a wave-wide search whose lanes all run this
+// body for the same cavity).
+template <class DV = DvScan>
 __host__ __device__ inline bool cav_retri_one(const Cons& c, const int2* pieces, const Win* wins, int wi, int* scr,
                                               unsigned epoch)
 {
@@ -217,6 +241,8 @@ __host__ __device__ inline bool cav_retri_one(const Cons& c, const int2* pieces,
         int* Rb = Lc + (K + 1);           // K+1 outer ids, right edges (walk order)
         int* Rc = Rb + (K + 1);           // K+1
         int* stk = Rc + (K + 1);          // 4 * (K+1)
+        int* PX = stk + 4 * (K + 1);      // K+2: one pseudo-polygon's vertex coordinates
+        int* PY = PX + (K + 2);           // K+2
         int2 pc = pieces[w.piece];
         int a = pc.x, b = pc.y;
         int2 PA = m.pts[a], PB = m.pts[b];
@@ -290,6 +316,15 @@ __host__ __device__ inline bool cav_retri_one(const Cons& c, const int2* pieces,
         int topL = -1, topR = -1;
         for (int side = 0; side < 2; side++) {
             int M = side == 0 ? nl : nr;
+            // the polygon's coordinates in order, so the Delaunay-vertex scans
+            // read consecutive words instead of a vertex id and then its point
+            for (int i = 0; i <= M + 1; i++) {
+                const int v = side == 0 ? (i == 0 ? a : (i == M + 1 ? b : Lv[i - 1]))
+                                        : (i == 0 ? b : (i == M + 1 ? a : Rv[M - i]));
+                const int2 P = m.pts[v];
+                PX[i] = P.x;
+                PY[i] = P.y;
+            }
             int sp = 0;
             stk[0] = 1;
             stk[1] = M + 1;
@@ -306,16 +341,7 @@ __host__ __device__ inline bool cav_retri_one(const Cons& c, const int2* pieces,
                 auto bnd_t = [&](int i) -> int { return side == 0 ? Lb[i] : Rb[M - i]; };
                 auto bnd_c = [&](int i) -> int { return side == 0 ? Lc[i] : Rc[M - i]; };
                 int A = poly(lo - 1), B = poly(hi);
-                int2 PA2 = m.pts[A], PB2 = m.pts[B];
-                int ci = lo;
-                int2 PC = m.pts[poly(lo)];
-                for (int i = lo + 1; i < hi; i++) {
-                    int2 PI = m.pts[poly(i)];
-                    if (incircle(PA2, PB2, PC, PI) > 0) {
-                        ci = i;
-                        PC = PI;
-                    }
-                }
+                const int ci = DV::find(PX, PY, lo, hi, make_int2(PX[lo - 1], PY[lo - 1]), make_int2(PX[hi], PY[hi]));
                 int C = poly(ci);
                 int T = slots[e++];
                 int3 nt = make_int3(A, B, C);

@@ -27,10 +27,12 @@ import importlib  # noqa: E402
 L = importlib.import_module("gcdt_amd.lib")  # signature table only; libgcdt.so is not loaded
 
 LIB = os.path.join(HERE, "_build", "libgcdt_emu.so")
-# variant whose per-cell reorder ranks buckets of up to 3 elements lane by
-# lane, sorts 4-8 in a workgroup and keeps the order of larger ones, so all
-# three paths run on small meshes (the defaults are 256 and 2048)
-CELLCAP = (("GCDT_CELL_RANK_MAX=3", "GCDT_CELL_BLOCK_MAX=8"),
+# variant with small caps, so the paths for large cases run on small meshes:
+# the per-cell reorder ranks buckets of up to 3 elements lane by lane, sorts
+# 4-8 in a workgroup and merge-sorts larger ones (the defaults are 256 and
+# 2048), and cavities of more than 4 crossed triangles are retriangulated by a
+# whole wave (the default is 64)
+CELLCAP = (("GCDT_CELL_RANK_MAX=3", "GCDT_CELL_BLOCK_MAX=8", "GCDT_CAV_WAVE_K=4"),
           os.path.join(HERE, "_build", "libgcdt_emu_cellcap.so"))
 _lib = None
 

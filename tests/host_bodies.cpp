@@ -379,7 +379,7 @@ extern "C" int host_solve(const int* xy, int N, const int* consp, int C, int* tr
                 ncross[i] = k;
                 tot += k;
             }
-            std::vector<int> scr(11 * tot + 16 * Q.size() + 64);
+            std::vector<int> scr(13 * tot + 20 * Q.size() + 64);
             std::vector<Win> wins;
             std::vector<int> Q2;
             int cursor = 0;

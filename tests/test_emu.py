@@ -165,9 +165,11 @@ def test_same_output_under_every_interleaving(case):
 
 def test_per_cell_reorder_paths_keep_valid_meshes():
     """With the per-cell reorder's caps at 3 and 8 elements, the lane-rank,
-    the workgroup and the large-cell merge paths all run on small meshes:
+    the workgroup and the large-cell merge paths all run on small meshes, and
+    with the wave cavity cap at 4 most cavities are retriangulated by a wave:
     every result must still pass the validator (a reorder that lost or
-    doubled an element would corrupt the point or leaf order)."""
+    doubled an element would corrupt the point or leaf order; a wrong
+    Delaunay vertex would leave a non-Delaunay edge)."""
     lib = emu.build(emu.CELLCAP)
     for case in ({"kind": "lattice", "k": 64, "c": 8, "max_len": 12}, {"kind": "uniform", "n": 6000, "c": 40},
                  {"kind": "clustered", "n": 8000, "circles": 4},
@@ -191,3 +193,21 @@ def test_large_cells_are_deterministic_too(case):
         assert rc == 0 and out is not None and out["valid"], (case, out, err)
         digests.add(out["digest"])
     assert len(digests) == 1
+
+
+@pytest.mark.parametrize("case", [{"kind": "lattice", "k": 64, "c": 8, "max_len": 12},
+                                  {"kind": "uniform", "n": 6000, "c": 40},
+                                  {"kind": "batch", "sizes": [64, 700, 300], "seed": 9}],
+                         ids=["lattice", "uniform", "batch"])
+def test_small_cap_paths_give_the_default_arrays(case):
+    """Every path of the per-cell reorder places the same order, and the wave
+    retriangulation picks the same Delaunay vertices as one thread's scan
+    (cocircular lattice included): the small-cap build's arrays equal the
+    default build's, bit for bit."""
+    small = emu.build(emu.CELLCAP)
+    digests = []
+    for lib in (None, small):
+        rc, out, err = emu.run_in_child(dict(case, digest=1), seed=2, conc=8, yield_atomics=1, lib=lib)
+        assert rc == 0 and out is not None and out["valid"], (case, out, err)
+        digests.append(out["digest"])
+    assert digests[0] == digests[1]
