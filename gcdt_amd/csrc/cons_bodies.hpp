@@ -62,20 +62,43 @@ __host__ __device__ inline void mark_pair(const Cons& c, int t, int k)
     }
 }
 
-// Cross the edge (r,l) of t: neighbour n and its apex q.  Returns the
-// constraint bit of the crossed edge.
-__host__ __device__ inline int cross_edge(const Mesh& m, int t, int r, int l, int& n, int& q)
-{
-    int3 tv = m.tri[t];
-    int k = slot_other(tv, r, l);
-    int cf = (m.cflag[t] >> k) & 1;
-    n = A3(m, t, k);
-    if (n >= 0) {
-        int3 nv = m.tri[n];
-        q = get3(nv, slot_other(nv, r, l));
+// A walk along a segment through the triangles it crosses.  It holds the
+// current triangle's vertices, neighbours and constraint bits, so crossing an
+// edge costs two dependent loads: the neighbour's vertices, then its
+// neighbours, its constraint bits and the apex's point together.
+struct Cross {
+    int t = -1;
+    int3 tv, ta;
+    int cf = 0;
+    __host__ __device__ void start(const Mesh& m, int t0)
+    {
+        t = t0;
+        tv = m.tri[t0];
+        ta = m.adj[t0];
+        cf = m.cflag[t0];
     }
-    return cf;
-}
+    // Crosses the edge (r,l) of t: n is the neighbour, q its apex and P the
+    // apex's point; the walk moves to n when n >= 0.  Returns the constraint
+    // bit of the crossed edge.
+    __host__ __device__ int step(const Mesh& m, int r, int l, int& n, int& q, int2& P)
+    {
+        const int k = slot_other(tv, r, l);
+        const int bit = (cf >> k) & 1;
+        n = get3(ta, k);
+        if (n >= 0) {
+            const int3 nv = m.tri[n];
+            q = get3(nv, slot_other(nv, r, l));
+            const int3 na = m.adj[n];
+            const int ncf = m.cflag[n];
+            P = m.pts[q];
+            t = n;
+            tv = nv;
+            ta = na;
+            cf = ncf;
+        }
+        return bit;
+    }
+};
 
 // ---------------------------------------------------------------- split pass
 
@@ -106,13 +129,15 @@ __host__ __device__ inline int split_walk(const Cons& c, int a, int b, int2* out
             continue;
         }
         int rr = get3(tv, nx3(s)), ll = get3(tv, pv3(s));
-        int tt = t;
+        Cross wk;
+        wk.start(m, t);
         int next = -1;
         for (int g2 = 0; g2 < (1 << 26); g2++) {
             int n, q;
+            int2 PQ;
             // a constrained edge crossed: an error unless crossings are
             // resolved later (the split only looks for vertices on the segment)
-            if (cross_edge(m, tt, rr, ll, n, q) && !c.steiner) {
+            if (wk.step(m, rr, ll, n, q, PQ) && !c.steiner) {
                 at_or(c.err, MERR_CROSSING);
                 return np;
             }
@@ -124,14 +149,13 @@ __host__ __device__ inline int split_walk(const Cons& c, int a, int b, int2* out
                 next = b;
                 break;
             }
-            int o = orient2d(PA, PB, m.pts[q]);
+            int o = orient2d(PA, PB, PQ);
             if (o == 0) {
                 next = q;
                 break;
             }
             if (o > 0) ll = q;
             else rr = q;
-            tt = n;
         }
         if (next < 0) {
             at_or(c.err, MERR_WALK);
@@ -171,9 +195,12 @@ __host__ __device__ inline int cav_walk(const Cons& c, int a, int b, bool mark, 
     int rr = get3(tv, nx3(s)), ll = get3(tv, pv3(s));
     int cnt = 1;
     f(t);
+    Cross wk;
+    wk.start(m, t);
     for (int g = 0; g < (1 << 26); g++) {
         int n, q;
-        if (cross_edge(m, t, rr, ll, n, q)) {
+        int2 PQ;
+        if (wk.step(m, rr, ll, n, q, PQ)) {
             if (c.steiner) return -2;
             at_or(c.err, MERR_CROSSING);
             return -1;
@@ -182,11 +209,10 @@ __host__ __device__ inline int cav_walk(const Cons& c, int a, int b, bool mark, 
         f(n);
         cnt++;
         if (q == b) return cnt;
-        int o = orient2d(PA, PB, m.pts[q]);
+        int o = orient2d(PA, PB, PQ);
         if (o == 0) return -1;
         if (o > 0) ll = q;
         else rr = q;
-        t = n;
     }
     return -1;
 }
@@ -265,36 +291,40 @@ __host__ __device__ inline bool cav_retri_one(const Cons& c, const int2* pieces,
         Rb[nrb] = A3(m, t, pv3(s));  // (a,u) opposite w
         Rc[nrb++] = (ft >> pv3(s)) & 1;
         bool bad = false;
+        Cross wk;
+        wk.start(m, t);
         for (;;) {
             int n, q;
-            cross_edge(m, t, rr, ll, n, q);
+            int2 PQ;
+            wk.step(m, rr, ll, n, q, PQ);
             if (n < 0 || ns >= K) {
                 bad = true;
                 break;
             }
             slots[ns++] = n;
-            int3 nv = m.tri[n];
-            int fn = m.cflag[n];
+            // the walk now holds n's vertices, neighbours and constraint bits
+            const int3 nv = wk.tv, na = wk.ta;
+            const int fn = wk.cf;
             int kr = slot_of(nv, rr), kl = slot_of(nv, ll);
             if (kr < 0 || kl < 0) {
                 bad = true;
                 break;
             }
             if (q == b) {
-                Lb[nlb] = A3(m, n, kr);  // (ll,b) opposite rr
+                Lb[nlb] = get3(na, kr);  // (ll,b) opposite rr
                 Lc[nlb++] = (fn >> kr) & 1;
-                Rb[nrb] = A3(m, n, kl);  // (rr,b) opposite ll
+                Rb[nrb] = get3(na, kl);  // (rr,b) opposite ll
                 Rc[nrb++] = (fn >> kl) & 1;
                 break;
             }
-            int o = orient2d(PA, PB, m.pts[q]);
+            int o = orient2d(PA, PB, PQ);
             if (o > 0) {  // boundary (ll,q) opposite rr
-                Lb[nlb] = A3(m, n, kr);
+                Lb[nlb] = get3(na, kr);
                 Lc[nlb++] = (fn >> kr) & 1;
                 Lv[nl++] = q;
                 ll = q;
             } else if (o < 0) {  // boundary (rr,q) opposite ll
-                Rb[nrb] = A3(m, n, kl);
+                Rb[nrb] = get3(na, kl);
                 Rc[nrb++] = (fn >> kl) & 1;
                 Rv[nr++] = q;
                 rr = q;
@@ -302,7 +332,6 @@ __host__ __device__ inline bool cav_retri_one(const Cons& c, const int2* pieces,
                 bad = true;
                 break;
             }
-            t = n;
         }
         if (bad || ns != K || nl + nr != K) {
             at_or(c.err, MERR_CAVITY);
