@@ -158,6 +158,33 @@ def solve_carved(pts_grid: np.ndarray, cons: np.ndarray):
     return tris, cflag, inside, st.as_dict()
 
 
+CHECK_BITS = {1: "vertex ids", 2: "orientation", 4: "adjacency", 8: "constraint bits", 16: "local Delaunay"}
+
+
+def check_mesh(pts_grid: np.ndarray, tris: np.ndarray, adj: np.ndarray, cflag: np.ndarray | None = None,
+               delaunay: bool = True):
+    """gcdt_check_mesh on host arrays in the caller's ids (pts_grid without the
+    four corners, which are appended as vertices n..n+3).  Returns (failed
+    check bits, first failing triangle or -1)."""
+    p = np.ascontiguousarray(pts_grid, dtype=np.int32).reshape(-1, 2)
+    R = (1 << 30) - 1
+    pc = np.concatenate([p, np.array([[0, 0], [R, 0], [R, R], [0, R]], np.int32)])
+    t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+    a = np.ascontiguousarray(adj, dtype=np.int32).reshape(-1, 3)
+    bufs = [DeviceBuffer.from_array(pc), DeviceBuffer.from_array(t), DeviceBuffer.from_array(a)]
+    if cflag is not None:
+        bufs.append(DeviceBuffer.from_array(np.ascontiguousarray(cflag, dtype=np.int32)))
+    try:
+        f, first = ctypes.c_int(0), ctypes.c_int(0)
+        check(lib().gcdt_check_mesh(bufs[0].ptr, len(pc), bufs[1].ptr, bufs[2].ptr,
+                                    bufs[3].ptr if cflag is not None else None, len(t), int(delaunay),
+                                    ctypes.byref(f), ctypes.byref(first), None), "gcdt_check_mesh")
+    finally:
+        for b in bufs:
+            b.free()
+    return f.value, first.value
+
+
 def pack_instances(instances):
     """[(pts_grid (n_i,2), cons (c_i,2) local ids)] -> flat arrays + offsets."""
     pts = [np.ascontiguousarray(p, dtype=np.int32).reshape(-1, 2) for p, _ in instances]

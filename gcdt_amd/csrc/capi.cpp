@@ -317,8 +317,10 @@ void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, i
     aA.release();
     sA.release();
     clA.release();
+    validate_mesh(c, "the sample mesh's insertion", Mesh{pts, tB.p, aB.p, nullptr, nullptr}, N + 4, Ts, false);
     int fr = 0;
     st.flips += flip_rounds(c, Mesh{pts, tB.p, aB.p, nullptr, nullptr}, Ts, nullptr, 0, &fr);
+    validate_mesh(c, "the sample mesh's flips", Mesh{pts, tB.p, aB.p, nullptr, nullptr}, N + 4, Ts, true);
     st.ms_sample = t1.ms();
     // ---- 2. bitmap over the sample Delaunay mesh
     Timer t2(c.st);
@@ -364,6 +366,7 @@ void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, i
     Timer t4(c.st);
     int f2 = 0;
     st.flips += flip_rounds_records(c, rec.p, Mesh{pts, tris_out, adj_out, nullptr, nullptr}, T, &f2);
+    validate_mesh(c, "the flips", Mesh{pts, tris_out, adj_out, nullptr, nullptr}, N + 4, T, true);
     rec.release();
     st.flip_rounds = fr + f2;
     st.ms_flip = t4.ms();
@@ -396,6 +399,7 @@ void solve_impl(Ctx& c, const int2* pts_in, int N, const int2* cons_in, int C, i
             st.flips += flip_rounds(c, m, T, tl.p, nt, &f3);
             st.flip_rounds += f3;
         }
+        validate_mesh(c, "the constraints", m, N + 4, T, true);
         if (steiner) {
             tl.release();
             touched.release();
@@ -872,8 +876,10 @@ void batch_impl(const gcdt_int2* points, const std::vector<long long>& poff, con
     aA.release();
     sA.release();
     clA.release();
+    validate_mesh(c, "the sample meshes' insertion", Mesh{pts.p, tB.p, aB.p, nullptr, nullptr}, nv, Ts, false);
     int fr = 0;
     bs.flips += flip_rounds(c, Mesh{pts.p, tB.p, aB.p, nullptr, nullptr}, Ts, nullptr, 0, &fr);
+    validate_mesh(c, "the sample meshes' flips", Mesh{pts.p, tB.p, aB.p, nullptr, nullptr}, nv, Ts, true);
     // ---- 2. locate the other points through a per-instance bitmap of the
     //         sample meshes, then insert them
     int *bo = nullptr, *bt = nullptr, nbt = 0;
@@ -917,6 +923,7 @@ void batch_impl(const gcdt_int2* points, const std::vector<long long>& poff, con
     Timer t2(c.st);
     int fr2 = 0;
     bs.flips += flip_rounds_records(c, rec.p, Mesh{pts.p, tri.p, adj.p, nullptr, nullptr}, T, &fr2);
+    validate_mesh(c, "the flips", Mesh{pts.p, tri.p, adj.p, nullptr, nullptr}, nv, T, true);
     rec.release();
     bs.flip_rounds = fr + fr2;
     bs.ms_flip = t2.ms();
@@ -939,6 +946,7 @@ void batch_impl(const gcdt_int2* points, const std::vector<long long>& poff, con
             bs.flips += flip_rounds(c, m, T, tl.p, ntl, &f3);
             bs.flip_rounds += f3;
         }
+        validate_mesh(c, "the constraints", m, nv, T, true);
     }
     bs.ms_constraints = t3.ms();
     batch_out(c, tri.p, T, vinst.p, perm.p, d_poff.p, d_toff.p, NP, K, (int3*)tris_out);
@@ -1066,6 +1074,9 @@ extern "C" int gcdt_set_tunable(const char* name, int value)
     } else if (n == "single_tester") {
         if ((rc = range(0, 1)) || (rc = cleared())) return rc;
         Tunables::get().single_tester = value;
+    } else if (n == "validate") {
+        if ((rc = range(0, 1)) || (rc = cleared())) return rc;
+        Tunables::get().validate = value;
     } else if (n == "grid_cap") {
         if ((rc = range(256, 65536)) || (rc = cleared())) return rc;
         g_grid_cap = value;
@@ -1086,9 +1097,23 @@ extern "C" int gcdt_get_tunable(const char* name, int* value)
     if (n == "tail_threshold") *value = Tunables::get().tail_threshold;
     else if (n == "pivot") *value = Tunables::get().pivot;
     else if (n == "single_tester") *value = Tunables::get().single_tester;
+    else if (n == "validate") *value = Tunables::get().validate;
     else if (n == "grid_cap") *value = g_grid_cap;
     else if (n == "flip_items") *value = Tunables::get().flip_items;
     else if (n == "ins_items") *value = Tunables::get().ins_items;
     else return fail(GCDT_ERR_INVALID, "unknown tunable " + n);
     return GCDT_OK;
+}
+
+extern "C" int gcdt_check_mesh(const gcdt_int2* points, long long nv, const gcdt_int3* tris, const gcdt_int3* adj,
+                               const int* cflag, int T, int delaunay, int* failed, int* first_bad, void* stream)
+{
+    API_BEGIN
+    if (!points || nv <= 0 || T < 0 || (T > 0 && (!tris || !adj)) || !failed || !first_bad)
+        return fail(GCDT_ERR_INVALID, "bad arguments");
+    Ctx c((hipStream_t)stream);
+    check_mesh(c, (const int2*)points, nv, (const int3*)tris, (const int3*)adj, cflag, T, delaunay != 0, failed,
+               first_bad);
+    return GCDT_OK;
+    API_END
 }

@@ -27,7 +27,7 @@ bool tunable_cleared(const std::string& name, int v)
         return v == 0 || v == 1024 || v == 2048 || v == 4096 || v == 8192 || v == 16384 || v == (1 << 20);
     if (name == "grid_cap") return v == 2048 || v == 4096 || v == 8192 || v == 16384;
     if (name == "flip_items" || name == "ins_items") return v == 1 || v == 2 || v == 4;
-    if (name == "pivot" || name == "single_tester") return v == 0 || v == 1;
+    if (name == "pivot" || name == "single_tester" || name == "validate") return v == 0 || v == 1;
     return false;
 }
 
@@ -49,6 +49,7 @@ Tunables& Tunables::get()
         env("GCDT_TAIL_THRESHOLD", "tail_threshold", &x.tail_threshold);
         env("GCDT_PIVOT", "pivot", &x.pivot);
         env("GCDT_SINGLE_TESTER", "single_tester", &x.single_tester);
+        env("GCDT_VALIDATE", "validate", &x.validate);
         env("GCDT_GRID_CAP", "grid_cap", &g_grid_cap);
         return x;
     }();
@@ -2510,6 +2511,87 @@ int flags_to_list(Ctx& c, const int* flags, int n, int* list)
     k_flag_scatter<<<grid_for(n), kBlock, 0, c.st>>>(flags, off.p, n, list);
     GCDT_LAUNCH_CHECK();
     return c.get(C_TOTAL);
+}
+
+// ================================================================== phase check
+
+// Per-triangle checks of a mesh between phases (the "validate" tunable):
+// bits of C_VAL name what failed, C_VAL_FIRST the lowest failing triangle.
+enum : int { VAL_IDS = 1, VAL_CW = 2, VAL_ADJ = 4, VAL_CFLAG = 8, VAL_DELAUNAY = 16 };
+
+__global__ __launch_bounds__(kBlock) void k_validate_mesh(const int2* pts, long long nv, const int3* tri,
+                                                          const int3* adj, const int* cflag, int T, int delaunay,
+                                                          int* bad, int* first)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < T; i += (long long)gridDim.x * blockDim.x) {
+        const int3 v = tri[i];
+        int f = 0;
+        if (v.x < 0 || v.y < 0 || v.z < 0 || v.x >= nv || v.y >= nv || v.z >= nv) {
+            f = VAL_IDS;
+        } else {
+            const int2 A = pts[v.x], B = pts[v.y], C = pts[v.z];
+            if (orient2d(A, B, C) <= 0) f |= VAL_CW;
+            const int3 a = adj[i];
+            const int cf = cflag ? cflag[i] : 0;
+            for (int k = 0; k < 3; k++) {
+                const int n = get3(a, k);
+                if (n < 0) continue;  // the bounding square's sides
+                if (n >= T) {
+                    f |= VAL_ADJ;
+                    continue;
+                }
+                const int u = get3(v, nx3(k)), w = get3(v, pv3(k));
+                const int3 nt = tri[n];
+                if (!has2(nt, u, w)) {
+                    f |= VAL_ADJ;
+                    continue;
+                }
+                const int j = slot_other(nt, u, w);
+                if (get3(adj[n], j) != (int)i) {
+                    f |= VAL_ADJ;
+                    continue;
+                }
+                const int ck = (cf >> k) & 1;
+                if (cflag && ck != ((cflag[n] >> j) & 1)) f |= VAL_CFLAG;
+                const int o = get3(nt, j);
+                if (delaunay && !ck && n > i && (o < 0 || o >= nv || incircle(A, B, C, pts[o]) > 0)) f |= VAL_DELAUNAY;
+            }
+        }
+        if (f) {
+            atomicOr(bad, f);
+            atomicMin(first, (int)i);
+        }
+    }
+}
+
+void check_mesh(Ctx& c, const int2* pts, long long nv, const int3* tri, const int3* adj, const int* cflag, int T,
+                bool delaunay, int* failed, int* first)
+{
+    *failed = 0;
+    *first = -1;
+    if (T <= 0) return;
+    c.set(C_VAL, 0);
+    c.set(C_VAL_FIRST, INT_MAX);
+    k_validate_mesh<<<grid_for(T), kBlock, 0, c.st>>>(pts, nv, tri, adj, cflag, T, delaunay ? 1 : 0, c.dcnt.p + C_VAL,
+                                                      c.dcnt.p + C_VAL_FIRST);
+    GCDT_LAUNCH_CHECK();
+    c.read(C_VAL, 2);
+    *failed = c.hcnt[C_VAL];
+    if (*failed) *first = c.hcnt[C_VAL_FIRST];
+}
+
+void validate_mesh(Ctx& c, const char* phase, const Mesh& m, long long nv, int T, bool delaunay)
+{
+    if (!Tunables::get().validate || T <= 0) return;
+    int f = 0, first = -1;
+    check_mesh(c, m.pts, nv, m.tri, m.adj, m.cflag, T, delaunay, &f, &first);
+    if (!f) return;
+    std::string what;
+    const char* names[] = {"vertex ids", "orientation", "adjacency", "constraint bits", "local Delaunay"};
+    for (int b = 0; b < 5; b++)
+        if (f & (1 << b)) what += std::string(what.empty() ? "" : ", ") + names[b];
+    throw Error(GCDT_ERR_INTERNAL, std::string("mesh check after ") + phase + " failed (" + what + "), first at triangle " +
+                                       std::to_string(first));
 }
 
 // ================================================================== leaves

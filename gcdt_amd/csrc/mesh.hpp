@@ -125,6 +125,8 @@ enum : int {
     C_STN_T = 24,     // crossing pass: triangles in use
     C_STN_S = 25,     // crossing pass: crossing vertices
     C_HUGE = 26,      // count passes: cells of more elements than one workgroup sorts
+    C_VAL = 27,       // mesh check: failed checks (VAL_* bits)
+    C_VAL_FIRST = 28, // mesh check: first failing triangle
     C_NUM = 32
 };
 
@@ -138,6 +140,7 @@ struct Tunables {
     int single_tester = 1;
     int flip_items = 1;  // items per thread of flip test/apply: 1, 2 or 4
     int ins_items = 1;   // items per thread of insertion split/relocate: 1, 2 or 4
+    int validate = 0;    // 1: check the mesh after every phase (validate_mesh)
     std::string env_error;
     static Tunables& get();
 };
@@ -253,6 +256,15 @@ int insert_rounds(Ctx& c, const Ins& s, DBuf<int>& pend, DBuf<int>& pend2, int n
                   int* rounds);
 // flip to (constrained) Delaunay; init list (nullptr = all T triangles)
 long long flip_rounds(Ctx& c, const Mesh& m, int T, const int* init, int ninit, int* rounds);
+// With the "validate" tunable on: checks m's T triangles over nv vertices
+// (counter-clockwise, symmetric adjacency, equal constraint bits on both
+// sides, and with delaunay every unconstrained edge locally Delaunay) and
+// fails with GCDT_ERR_INTERNAL naming `phase`.  Off: returns at once.
+void validate_mesh(Ctx& c, const char* phase, const Mesh& m, long long nv, int T, bool delaunay);
+// The checks themselves: *failed gets the failed-check bits (VAL_*), *first the
+// lowest failing triangle (-1 if none).
+void check_mesh(Ctx& c, const int2* pts, long long nv, const int3* tri, const int3* adj, const int* cflag, int T,
+                bool delaunay, int* failed, int* first);
 // whole-mesh flip rounds on records already packed (filter_leaves_*_rec);
 // the result is unpacked into m.tri / m.adj (and m.cflag when set)
 long long flip_rounds_records(Ctx& c, FRec* rec, const Mesh& m, int T, int* rounds);

@@ -114,6 +114,7 @@ SIGNATURES = {
     "gcdt_solve_flags": ([vp, i32, vp, i32, vp, vp, vp, pint, ctypes.POINTER(Stats), vp], i32),
     "gcdt_solve_noalloc": ([vp, i32, vp, i32, vp, vp, pint, ctypes.POINTER(Stats), vp], i32),
     "gcdt_carve": ([vp, vp, vp, i32, i32, vp, pint, vp], i32),
+    "gcdt_check_mesh": ([vp, ctypes.c_int64, vp, vp, vp, i32, i32, pint, pint, vp], i32),
     "gcdt_solve_steiner": ([vp, i32, vp, i32, i32, vp, vp, vp, vp, pint, pint, ctypes.POINTER(Stats), vp], i32),
     "gcdt_solve_batch": ([vp, ctypes.POINTER(ctypes.c_int64), i32, vp, ctypes.POINTER(ctypes.c_int64), vp,
                           ctypes.POINTER(BatchStats), vp, vp], i32),

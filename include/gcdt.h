@@ -260,6 +260,19 @@ int gcdt_solve_steiner(const gcdt_int2* points, int N, const gcdt_int2* cons, in
 int gcdt_carve(const gcdt_int3* tris, const gcdt_int3* adj, const int* cflag, int T, int numPoints, int* inside,
                int* num_inside, void* stream);
 
+/* Mesh check on the device (no counterpart in the reference, which only
+ * prints debug counts; the solves run it after every phase when the
+ * "validate" tunable is 1): points holds nv vertices (corners included),
+ * tris/adj T triangles with -1 for the bounding square's sides, cflag (or
+ * NULL) the constraint bits of gcdt_solve_flags.  *failed gets a bit set per
+ * failed check -- 1 vertex ids, 2 orientation (not counter-clockwise),
+ * 4 adjacency (a neighbour that does not name the triangle back across the
+ * same edge), 8 constraint bits differing across an edge, 16 an unconstrained
+ * edge not locally Delaunay (checked when delaunay is 1) -- and *first_bad the
+ * lowest failing triangle (-1 if none).  Returns GCDT_OK whatever it finds. */
+int gcdt_check_mesh(const gcdt_int2* points, long long nv, const gcdt_int3* tris, const gcdt_int3* adj,
+                    const int* cflag, int T, int delaunay, int* failed, int* first_bad, void* stream);
+
 /* ------------------------------------------------------------ batches */
 
 typedef struct {
@@ -306,7 +319,11 @@ int gcdt_solve_batch(const gcdt_int2* points, const int64_t* point_offsets, int 
  * "grid_cap" [256, 65536] (workgroups of a grid-stride launch),
  * "flip_items" / "ins_items" (1, 2 or 4 list items per thread of the flip
  * test/apply and insertion split/relocate launches), "pivot"
- * (0 random, 1 central-first, the default), "single_tester" (0/1).  Out-of-range values
+ * (0 random, 1 central-first, the default), "single_tester" (0/1), "validate"
+ * (0/1: 1 checks the mesh on the device after every phase -- orientation,
+ * adjacency symmetry, constraint bits on both sides of an edge, local Delaunay
+ * after the flip phases -- and fails the solve with GCDT_ERR_INTERNAL naming
+ * the phase; environment GCDT_VALIDATE).  Out-of-range values
  * fail with GCDT_ERR_INVALID, and so do in-range values that no GPU parity
  * test covers (the same rule applies to the GCDT_* environment overrides: a
  * refused one makes every solve fail).  gcdt_amd/tune.py picks tail_threshold,

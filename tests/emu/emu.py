@@ -225,6 +225,76 @@ def run_case(case: dict) -> dict:
             if v["valid"] and v["unique"]:
                 res["unique_mismatch"] += int(not np.array_equal(O.canonical(t0), O.canonical(t1)))
         return res
+    if kind == "check":
+        # gcdt_check_mesh on a solved mesh and on copies with one planted
+        # mistake each; returns the failed-check bits per copy
+        p, c = gen.uniform_cdt(n=case["n"], c=case.get("c", 0), seed_pts=3, seed_cons=4)
+        g = gen.to_grid(p)
+        tris, adj, _ = solve(g, c)
+        R = (1 << 30) - 1
+        pc = np.concatenate([g, np.array([[0, 0], [R, 0], [R, R], [0, R]], np.int32)])
+
+        def bits(t, a, delaunay=1):
+            bufs = [Buf.of(pc), Buf.of(np.ascontiguousarray(t)), Buf.of(np.ascontiguousarray(a))]
+            f, first = ctypes.c_int(0), ctypes.c_int(0)
+            check(load().gcdt_check_mesh(bufs[0].ptr, len(pc), bufs[1].ptr, bufs[2].ptr, None, len(t), delaunay,
+                                         ctypes.byref(f), ctypes.byref(first), None), "gcdt_check_mesh")
+            for b in bufs:
+                b.free()
+            return f.value, first.value
+
+        res["clean"] = bits(tris, adj)
+        t2 = tris.copy()
+        t2[5, [1, 2]] = t2[5, [2, 1]]  # clockwise, and its neighbours no longer match
+        res["swapped"] = bits(t2, adj)
+        a2 = adj.copy()
+        k = int(np.argmax(a2[7] >= 0))
+        a2[7, k] = (a2[7, k] + 1) % len(tris)  # a neighbour that does not name 7 back
+        res["adjacency"] = bits(tris, a2)
+        a3 = adj.copy()
+        a3[9, 0] = len(tris) + 5  # out of range
+        res["range"] = bits(tris, a3)
+        # flip an interior Delaunay edge that is not cocircular: both new
+        # triangles stay valid and counter-clockwise but the edge is not
+        # locally Delaunay
+        from oracle.oracle import with_corners
+        P = with_corners(g)
+        done = False
+        for t in range(len(tris)):
+            for k in range(3):
+                n = int(adj[t, k])
+                if n <= t:
+                    continue
+                u, w = int(tris[t, (k + 1) % 3]), int(tris[t, (k + 2) % 3])
+                pv = int(tris[t, k])
+                o = [int(x) for x in tris[n] if x != u and x != w][0]
+                def orient(a, b, cc):
+                    return int((P[b][0] - P[a][0]) * (P[cc][1] - P[a][1]) - (P[b][1] - P[a][1]) * (P[cc][0] - P[a][0]))
+                if orient(pv, u, o) <= 0 or orient(pv, o, w) <= 0:
+                    continue  # not a convex quad
+                t3 = tris.copy()
+                t3[t] = (pv, u, o)
+                t3[n] = (pv, o, w)
+                # the flipped pair's own adjacency, rebuilt by brute force
+                a4 = adj.copy()
+                for x in (t, n):
+                    for kk in range(3):
+                        e = {int(t3[x, (kk + 1) % 3]), int(t3[x, (kk + 2) % 3])}
+                        m = [y for y in range(len(t3)) if y != x and e <= set(int(z) for z in t3[y])]
+                        a4[x, kk] = m[0] if m else -1
+                        for y in m:
+                            for jj in range(3):
+                                if {int(t3[y, (jj + 1) % 3]), int(t3[y, (jj + 2) % 3])} == e:
+                                    a4[y, jj] = x
+                f, first = bits(t3, a4)
+                if f == 16:
+                    res["nondelaunay"] = (f, first)
+                    res["nondelaunay_without_check"] = bits(t3, a4, 0)
+                    done = True
+                    break
+            if done:
+                break
+        return res
     if kind == "steiner":
         from tests import steiner_cases as SC
         spec = case["gen"]

@@ -659,6 +659,7 @@ int emu_set_tunable(const char* name, int value)
     else if (n == "grid_cap") gcdt::g_grid_cap = value;
     else if (n == "flip_items") t.flip_items = value;
     else if (n == "ins_items") t.ins_items = value;
+    else if (n == "validate") t.validate = value;
     else return 2;
     return 0;
 }

@@ -211,3 +211,31 @@ def test_small_cap_paths_give_the_default_arrays(case):
         assert rc == 0 and out is not None and out["valid"], (case, out, err)
         digests.append(out["digest"])
     assert digests[0] == digests[1]
+
+
+# ---------------------------------------------------------------- phase checks
+
+def test_mesh_check_catches_planted_mistakes():
+    """gcdt_check_mesh: a solved Delaunay mesh passes; one triangle turned
+    clockwise, one wrong or out-of-range neighbour, and one flipped convex
+    edge (valid, counter-clockwise, not locally Delaunay) are each caught with
+    their check bits and the lowest failing triangle."""
+    rc, out, err = emu.run_in_child({"kind": "check", "n": 300}, seed=1, conc=4, yield_atomics=1)
+    assert rc == 0 and out is not None, err
+    assert out["clean"] == [0, -1]
+    assert out["swapped"][0] & 2 and out["swapped"][0] & 4 and out["swapped"][1] <= 5
+    assert out["adjacency"][0] == 4 and out["adjacency"][1] <= 7
+    assert out["range"] == [4, 9]
+    assert out["nondelaunay"][0] == 16 and out["nondelaunay_without_check"] == [0, -1]
+
+
+@pytest.mark.parametrize("case", [{"kind": "uniform", "n": 6000, "c": 40},
+                                  {"kind": "lattice", "k": 48, "c": 20, "max_len": 12},
+                                  {"kind": "clustered", "n": 8000, "circles": 4},
+                                  {"kind": "batch", "sizes": [64, 700, 300, 2000], "seed": 9}],
+                         ids=["uniform", "lattice", "clustered", "batch"])
+def test_solves_pass_the_phase_checks(case):
+    """With the validate tunable on, the mesh is checked on the device after
+    every phase (sample mesh, its flips, the flips, the constraints); a
+    failed check would fail the solve."""
+    _run(dict(case, knobs={"validate": 1}), seed=6, conc=8, yield_atomics=1)
